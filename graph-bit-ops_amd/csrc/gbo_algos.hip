@@ -1,0 +1,516 @@
+// Graph algorithms on the bit-BSR engine: BFS (algorithm/bfs.cu), SSSP
+// (algorithm/sssp.cu), PageRank (algorithm/pr.cu), connected components
+// (algorithm/cc.cu, FastSV) and triangle counting (bsrbmm/tc.cu).
+//
+// The reference drivers launch five to nine kernels per iteration, fill
+// vectors with single-thread <<<1,1>>> kernels inside the timed loop and
+// copy a convergence scalar to the host each iteration.  Here every
+// iteration is one bsrbmv launch plus one fused element-wise kernel (update
+// of frontier/visited/levels, distances, scores or labels together with the
+// convergence reduction); the host reads one 8-byte scalar per iteration.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "gbo_device.hpp"
+#include "gbo_ops.hpp"
+
+namespace gbo {
+
+template <class T>
+__global__ __launch_bounds__(256) void k_fill(T* __restrict__ p, int64_t n, T v) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+template <class T>
+static int fill(T* p, int64_t n, T v, hipStream_t st) {
+  if (n <= 0) return GBO_OK;
+  const int64_t blocks = std::min<int64_t>((n + 255) / 256, 65536);
+  k_fill<T><<<(unsigned)blocks, 256, 0, st>>>(p, n, v);
+  GBO_LAUNCH_CHECK();
+  return GBO_OK;
+}
+
+// ------------------------------------------------------------------ BFS
+template <int BS>
+__global__ __launch_bounds__(256) void k_bfs_init(word_t<BS>* f, word_t<BS>* vis, int32_t* level, int32_t nbr,
+                                                  int32_t n, int32_t source) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nbr) {
+    const word_t<BS> w = (i == source / BS) ? (word_t<BS>)(1ull << (BS - 1 - (source % BS))) : (word_t<BS>)0;
+    f[i] = w;
+    vis[i] = w;
+  }
+  if (i < n) level[i] = (i == source) ? 0 : -1;
+}
+
+// visited |= next; level[v] = lvl for each newly discovered v.
+template <int BS>
+__global__ __launch_bounds__(256) void k_bfs_update(const word_t<BS>* __restrict__ next, word_t<BS>* __restrict__ vis,
+                                                    int32_t* __restrict__ level, int32_t nbr, int32_t n, int32_t lvl) {
+  const int64_t J = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (J >= nbr) return;
+  uint64_t w = (uint64_t)next[J];
+  if (!w) return;
+  vis[J] = (word_t<BS>)((uint64_t)vis[J] | w);
+  while (w) {
+    const int p = __builtin_ctzll(w);
+    w &= w - 1;
+    const int64_t v = J * BS + (BS - 1 - p);
+    if (v < n) level[v] = lvl;
+  }
+}
+
+template <int BS>
+static int bfs_t(const gbo_bsr* A, int32_t source, int32_t* d_level, int64_t* sizes, int32_t max_levels,
+                 int32_t* nlevels, hipStream_t st) {
+  using W = word_t<BS>;
+  const int32_t nbr = A->nblockrows, n = A->nrows;
+  W *f = nullptr, *nx = nullptr, *vis = nullptr;
+  int64_t* d_cnt = nullptr;
+  GBO_HIP_CHECK(hipMallocAsync((void**)&f, sizeof(W) * (size_t)std::max(nbr, 1) * 3 + 64, st));
+  nx = f + std::max(nbr, 1);
+  vis = nx + std::max(nbr, 1);
+  GBO_HIP_CHECK(hipMallocAsync((void**)&d_cnt, sizeof(int64_t), st));
+  PlanView pv;
+  int rc = acquire_plan(A, &pv, st);
+  if (rc) return rc;
+  const unsigned blocks_n = (unsigned)((std::max<int64_t>(n, nbr) + 255) / 256);
+  k_bfs_init<BS><<<blocks_n, 256, 0, st>>>(f, vis, d_level, nbr, n, source);
+  GBO_LAUNCH_CHECK();
+  SpmvArgs a{};
+  a.rowptr = A->rowptr;
+  a.colind = A->colind;
+  a.tiles = A->tiles;
+  a.nrows = A->nrows;
+  a.ncols = A->ncols;
+  a.nbr = nbr;
+  a.T = A->nblocks;
+  a.plan = pv.plan;
+  a.chunk = pv.chunk;
+  a.nchunks = pv.nchunks;
+  a.count = d_cnt;
+  int32_t lvl = 0;
+  if (sizes && max_levels > 0) sizes[0] = 1;
+  while (true) {
+    GBO_HIP_CHECK(hipMemsetAsync(d_cnt, 0, sizeof(int64_t), st));
+    a.xbits = f;
+    a.visited = vis;
+    a.out = nx;
+    rc = spmv_launch(BS, OP_BFS, a, st);
+    if (rc) break;
+    int64_t cnt = 0;
+    GBO_HIP_CHECK(hipMemcpyAsync(&cnt, d_cnt, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    GBO_HIP_CHECK(hipStreamSynchronize(st));
+    if (cnt == 0) break;
+    ++lvl;
+    if (sizes && lvl < max_levels) sizes[lvl] = cnt;
+    k_bfs_update<BS><<<(unsigned)((nbr + 255) / 256), 256, 0, st>>>(nx, vis, d_level, nbr, n, lvl);
+    GBO_LAUNCH_CHECK();
+    std::swap(f, nx);
+  }
+  if (nlevels) *nlevels = lvl + 1;
+  release_plan(&pv, st);
+  W* base = std::min(std::min(f, nx), vis);
+  (void)hipFreeAsync(base, st);
+  (void)hipFreeAsync(d_cnt, st);
+  GBO_HIP_CHECK(hipStreamSynchronize(st));
+  return rc;
+}
+
+// ------------------------------------------------------------------ SSSP
+// dist = min(dist, cand); the improved vertices form the next frontier: their
+// distance goes to `fr` (INF elsewhere) and their block column is marked
+// active so the next min-plus product only reads tiles of active columns.
+__global__ __launch_bounds__(256) void k_sssp_update(const uint32_t* __restrict__ cand, uint32_t* __restrict__ dist,
+                                                     uint32_t* __restrict__ fr, unsigned long long* __restrict__ active,
+                                                     int32_t n, int bs, unsigned long long* __restrict__ changed) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int improved = 0;
+  if (i < n) {
+    const uint32_t c = cand[i], d = dist[i];
+    if (c < d) {
+      dist[i] = c;
+      fr[i] = c;
+      improved = 1;
+      const int64_t J = i / bs;
+      atomicOr(&active[J >> 6], 1ull << (J & 63));
+    } else {
+      fr[i] = kInfU32;
+    }
+  }
+  const uint64_t b = __ballot(improved);
+  if (lane_id() == 0 && b) atomicAdd(changed, (unsigned long long)__popcll(b));
+}
+
+__global__ void k_sssp_init(uint32_t* dist, uint32_t* fr, unsigned long long* active, int32_t n, int bs, int32_t s) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    dist[i] = (i == s) ? 0u : kInfU32;
+    fr[i] = (i == s) ? 0u : kInfU32;
+  }
+  if (i == 0) {
+    const int64_t J = s / bs;
+    active[J >> 6] = 1ull << (J & 63);
+  }
+}
+
+// ------------------------------------------------------------------ PageRank
+// x[j] = alpha * p[j] / outdeg[j] (0 for vertices without out-edges) -- the
+// per-edge factor alpha*f1/nnz of bmv32_sparse_full (algorithm/bsrbmv.cu:
+// 1276) computed once per vertex.
+__global__ __launch_bounds__(256) void k_pr_scale(const float* __restrict__ p, const int32_t* __restrict__ outdeg,
+                                                  float alpha, float* __restrict__ x, int32_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    const int32_t d = outdeg[i];
+    x[i] = d > 0 ? alpha * p[i] / (float)d : 0.0f;
+  }
+}
+
+// Per-block partial sums of |a-b| in double; the partials are added in
+// block order by k_sum_partials, so the result is reproducible.
+__global__ __launch_bounds__(256) void k_l1_partial(const float* __restrict__ a, const float* __restrict__ b, int32_t n,
+                                                    double* __restrict__ part) {
+  __shared__ double sh[256];
+  double s = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    s += fabs((double)a[i] - (double)b[i]);
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+__global__ void k_sum_partials(const double* __restrict__ part, int np, double* out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    double s = 0.0;
+    for (int i = 0; i < np; ++i) s += part[i];
+    *out = s;
+  }
+}
+
+// ------------------------------------------------------------------ CC
+// FastSV hooking (Zhang, Azad, Hu; the GraphBLAST algorithm the reference
+// follows, algorithm/cc.cu:1018-1083): stochastic hooking f[f[u]] = min(.,
+// mngf[u]), aggressive hooking f[u] = min(f[u], mngf[u]) and shortcutting
+// f[u] = min(f[u], gf[u]), all as min-accumulating atomics.
+__global__ __launch_bounds__(256) void k_cc_hook(const int32_t* __restrict__ mngf, const int32_t* __restrict__ gf,
+                                                 int32_t* __restrict__ f, int32_t n) {
+  const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= n) return;
+  const int32_t m = mngf[u];
+  const int32_t fu = f[u];
+  if (m < fu) atomicMin(&f[fu], m);
+  const int32_t g = gf[u];
+  const int32_t best = m < g ? m : g;
+  atomicMin(&f[u], best);
+}
+
+// gf[u] = f[f[u]]; counts the vertices whose grandparent changed.
+__global__ __launch_bounds__(256) void k_cc_grand(const int32_t* __restrict__ f, int32_t* __restrict__ gf, int32_t n,
+                                                  unsigned long long* __restrict__ changed) {
+  const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int ch = 0;
+  if (u < n) {
+    const int32_t g = f[f[u]];
+    if (g != gf[u]) {
+      gf[u] = g;
+      ch = 1;
+    }
+  }
+  const uint64_t b = __ballot(ch);
+  if (lane_id() == 0 && b) atomicAdd(changed, (unsigned long long)__popcll(b));
+}
+
+// f[u] = f[f[u]] until every vertex points at a root (final star shape).
+__global__ __launch_bounds__(256) void k_cc_jump(int32_t* __restrict__ f, int32_t n, unsigned long long* changed) {
+  const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int ch = 0;
+  if (u < n) {
+    const int32_t p = f[u], gp = f[p];
+    if (gp != p) {
+      f[u] = gp;
+      ch = 1;
+    }
+  }
+  const uint64_t b = __ballot(ch);
+  if (lane_id() == 0 && b) atomicAdd(changed, (unsigned long long)__popcll(b));
+}
+
+__global__ void k_iota(int32_t* a, int32_t* b, int32_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    a[i] = (int32_t)i;
+    b[i] = (int32_t)i;
+  }
+}
+
+static int read_u64(unsigned long long* d, uint64_t* h, hipStream_t st) {
+  GBO_HIP_CHECK(hipMemcpyAsync(h, d, sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+  GBO_HIP_CHECK(hipStreamSynchronize(st));
+  return GBO_OK;
+}
+
+static int check_square(const gbo_bsr* A, const char* who) {
+  int rc = validate_bsr(A, 0, who);
+  if (rc) return rc;
+  if (A->nrows != A->ncols) return set_error(GBO_EINVAL, "%s: matrix must be square", who);
+  return GBO_OK;
+}
+
+}  // namespace gbo
+
+using namespace gbo;
+
+extern "C" {
+
+int gbo_bfs(const gbo_bsr* AT, int32_t source, int32_t* d_level, int64_t* h_level_sizes, int32_t max_levels,
+            int32_t* h_nlevels, gbo_stream_t stream) {
+  int rc = check_square(AT, "gbo_bfs");
+  if (rc) return rc;
+  if (source < 0 || source >= AT->nrows) return set_error(GBO_EINVAL, "gbo_bfs: source %d out of range", source);
+  if (!d_level) return set_error(GBO_EINVAL, "gbo_bfs: null level");
+  switch (AT->bs) {
+    case 8: return bfs_t<8>(AT, source, d_level, h_level_sizes, max_levels, h_nlevels, stream);
+    case 16: return bfs_t<16>(AT, source, d_level, h_level_sizes, max_levels, h_nlevels, stream);
+    case 32: return bfs_t<32>(AT, source, d_level, h_level_sizes, max_levels, h_nlevels, stream);
+    case 64: return bfs_t<64>(AT, source, d_level, h_level_sizes, max_levels, h_nlevels, stream);
+  }
+  return set_error(GBO_EUNSUPPORTED, "gbo_bfs: block size");
+}
+
+int gbo_sssp(const gbo_bsr* AT, const uint32_t* d_w, const int64_t* d_woff, int32_t source, uint32_t* d_dist,
+             int32_t* h_iters, gbo_stream_t stream) {
+  int rc = check_square(AT, "gbo_sssp");
+  if (rc) return rc;
+  const int32_t n = AT->nrows;
+  if (source < 0 || source >= n) return set_error(GBO_EINVAL, "gbo_sssp: source %d out of range", source);
+  if (!d_dist) return set_error(GBO_EINVAL, "gbo_sssp: null dist");
+  if ((d_w == nullptr) != (d_woff == nullptr)) return set_error(GBO_EINVAL, "gbo_sssp: weights need offsets");
+  hipStream_t st = stream;
+  const int bs = AT->bs;
+  const int64_t nact = ((int64_t)AT->nblockcols + 63) / 64;
+  uint32_t *fr = nullptr, *cand = nullptr;
+  unsigned long long *active = nullptr, *changed = nullptr;
+  GBO_HIP_CHECK(hipMallocAsync((void**)&fr, sizeof(uint32_t) * (size_t)std::max(n, 1) * 2, st));
+  cand = fr + std::max(n, 1);
+  GBO_HIP_CHECK(hipMallocAsync((void**)&active, sizeof(unsigned long long) * (size_t)(nact + 1) * 2, st));
+  changed = active + 2 * nact + 1;
+  unsigned long long* active2 = active + nact;
+  PlanView pv;
+  rc = acquire_plan(AT, &pv, st);
+  if (rc) return rc;
+  GBO_HIP_CHECK(hipMemsetAsync(active, 0, sizeof(unsigned long long) * nact, st));
+  const unsigned blocks_n = (unsigned)((n + 255) / 256);
+  k_sssp_init<<<blocks_n, 256, 0, st>>>(d_dist, fr, active, n, bs, source);
+  GBO_LAUNCH_CHECK();
+  SpmvArgs a{};
+  a.rowptr = AT->rowptr;
+  a.colind = AT->colind;
+  a.tiles = AT->tiles;
+  a.nrows = n;
+  a.ncols = AT->ncols;
+  a.nbr = AT->nblockrows;
+  a.T = AT->nblocks;
+  a.plan = pv.plan;
+  a.chunk = pv.chunk;
+  a.nchunks = pv.nchunks;
+  a.w = d_w;
+  a.woff = d_woff;
+  int32_t it = 0;
+  unsigned long long *act_cur = active, *act_next = active2;
+  while (true) {
+    a.xval = fr;
+    a.out = cand;
+    a.active = (const uint64_t*)act_cur;
+    rc = spmv_launch(bs, OP_MINPLUS, a, st);
+    if (rc) break;
+    GBO_HIP_CHECK(hipMemsetAsync(act_next, 0, sizeof(unsigned long long) * nact, st));
+    GBO_HIP_CHECK(hipMemsetAsync(changed, 0, sizeof(unsigned long long), st));
+    k_sssp_update<<<blocks_n, 256, 0, st>>>(cand, d_dist, fr, act_next, n, bs, changed);
+    GBO_LAUNCH_CHECK();
+    ++it;
+    uint64_t h = 0;
+    rc = read_u64(changed, &h, st);
+    if (rc || h == 0) break;
+    if (it > n + 1) {  // distances only decrease: cannot happen
+      rc = set_error(GBO_ENOCONV, "gbo_sssp: no convergence after %d iterations", it);
+      break;
+    }
+    std::swap(act_cur, act_next);
+  }
+  if (h_iters) *h_iters = it;
+  release_plan(&pv, st);
+  (void)hipFreeAsync(fr, st);
+  (void)hipFreeAsync(active, st);
+  GBO_HIP_CHECK(hipStreamSynchronize(st));
+  return rc;
+}
+
+int gbo_pagerank(const gbo_bsr* AT, const int32_t* d_outdeg, float alpha, double tol, int32_t max_iter, float* d_p,
+                 int32_t* h_iters, double* h_last_delta, gbo_stream_t stream) {
+  int rc = check_square(AT, "gbo_pagerank");
+  if (rc) return rc;
+  const int32_t n = AT->nrows;
+  if (!d_outdeg || !d_p) return set_error(GBO_EINVAL, "gbo_pagerank: null argument");
+  if (!(alpha >= 0.0f && alpha < 1.0f) || max_iter < 1) return set_error(GBO_EINVAL, "gbo_pagerank: bad alpha/max_iter");
+  if (n == 0) return GBO_OK;
+  hipStream_t st = stream;
+  float *x = nullptr, *y = nullptr, *partial = nullptr;
+  double* dpart = nullptr;
+  const int nparts = 1024;
+  GBO_HIP_CHECK(hipMallocAsync((void**)&x, sizeof(float) * (size_t)n * 2, st));
+  y = x + n;
+  GBO_HIP_CHECK(hipMallocAsync((void**)&dpart, sizeof(double) * (nparts + 1), st));
+  PlanView pv;
+  rc = acquire_plan(AT, &pv, st);
+  if (rc) return rc;
+  GBO_HIP_CHECK(hipMallocAsync((void**)&partial, sizeof(float) * 2 * (size_t)pv.nchunks * AT->bs, st));
+  rc = fill<float>(d_p, n, 1.0f / (float)n, st);
+  SpmvArgs a{};
+  a.rowptr = AT->rowptr;
+  a.colind = AT->colind;
+  a.tiles = AT->tiles;
+  a.nrows = n;
+  a.ncols = AT->ncols;
+  a.nbr = AT->nblockrows;
+  a.T = AT->nblocks;
+  a.plan = pv.plan;
+  a.chunk = pv.chunk;
+  a.nchunks = pv.nchunks;
+  a.partial = partial;
+  a.scale = 1.0f;
+  a.bias = (1.0f - alpha) / (float)n;
+  const unsigned blocks_n = (unsigned)((n + 255) / 256);
+  const unsigned lblocks = (unsigned)std::min<int64_t>(nparts, (n + 255) / 256);
+  int32_t it = 0;
+  double delta = 0.0;
+  float* p = d_p;
+  float* q = y;  // next iterate
+  while (rc == GBO_OK && it < max_iter) {
+    k_pr_scale<<<blocks_n, 256, 0, st>>>(p, d_outdeg, alpha, x, n);
+    GBO_LAUNCH_CHECK();
+    a.xval = x;
+    a.out = q;
+    rc = spmv_launch(AT->bs, OP_SUM, a, st);
+    if (rc) break;
+    k_l1_partial<<<lblocks, 256, 0, st>>>(q, p, n, dpart);
+    k_sum_partials<<<1, 64, 0, st>>>(dpart, (int)lblocks, dpart + nparts);
+    GBO_LAUNCH_CHECK();
+    GBO_HIP_CHECK(hipMemcpyAsync(&delta, dpart + nparts, sizeof(double), hipMemcpyDeviceToHost, st));
+    GBO_HIP_CHECK(hipStreamSynchronize(st));
+    ++it;
+    std::swap(p, q);
+    if (delta < tol) break;
+  }
+  if (rc == GBO_OK && p != d_p) GBO_HIP_CHECK(hipMemcpyAsync(d_p, p, sizeof(float) * n, hipMemcpyDeviceToDevice, st));
+  release_plan(&pv, st);
+  (void)hipFreeAsync(partial, st);
+  (void)hipFreeAsync(x, st);
+  (void)hipFreeAsync(dpart, st);
+  GBO_HIP_CHECK(hipStreamSynchronize(st));
+  if (h_iters) *h_iters = it;
+  if (h_last_delta) *h_last_delta = delta;
+  if (rc == GBO_OK && !(delta < tol))
+    return set_error(GBO_ENOCONV, "gbo_pagerank: not converged after %d iterations (delta %.3e)", it, delta);
+  return rc;
+}
+
+int gbo_cc(const gbo_bsr* A, int32_t* d_label, int32_t* h_iters, gbo_stream_t stream) {
+  int rc = check_square(A, "gbo_cc");
+  if (rc) return rc;
+  const int32_t n = A->nrows;
+  if (!d_label) return set_error(GBO_EINVAL, "gbo_cc: null label");
+  if (n == 0) return GBO_OK;
+  hipStream_t st = stream;
+  int32_t *gf = nullptr, *mngf = nullptr;
+  unsigned long long* changed = nullptr;
+  GBO_HIP_CHECK(hipMallocAsync((void**)&gf, sizeof(int32_t) * (size_t)n * 2, st));
+  mngf = gf + n;
+  GBO_HIP_CHECK(hipMallocAsync((void**)&changed, sizeof(unsigned long long), st));
+  int32_t* f = d_label;
+  const unsigned blocks_n = (unsigned)((n + 255) / 256);
+  k_iota<<<blocks_n, 256, 0, st>>>(f, gf, n);
+  GBO_LAUNCH_CHECK();
+  PlanView pv;
+  rc = acquire_plan(A, &pv, st);
+  if (rc) return rc;
+  SpmvArgs a{};
+  a.rowptr = A->rowptr;
+  a.colind = A->colind;
+  a.tiles = A->tiles;
+  a.nrows = n;
+  a.ncols = A->ncols;
+  a.nbr = A->nblockrows;
+  a.T = A->nblocks;
+  a.plan = pv.plan;
+  a.chunk = pv.chunk;
+  a.nchunks = pv.nchunks;
+  int32_t it = 0;
+  while (true) {
+    a.xval = gf;
+    a.out = mngf;
+    rc = spmv_launch(A->bs, OP_MIN, a, st);
+    if (rc) break;
+    k_cc_hook<<<blocks_n, 256, 0, st>>>(mngf, gf, f, n);
+    GBO_HIP_CHECK(hipMemsetAsync(changed, 0, sizeof(unsigned long long), st));
+    k_cc_grand<<<blocks_n, 256, 0, st>>>(f, gf, n, changed);
+    GBO_LAUNCH_CHECK();
+    ++it;
+    uint64_t h = 0;
+    rc = read_u64(changed, &h, st);
+    if (rc || h == 0) break;
+    if (it > n + 2) {
+      rc = set_error(GBO_ENOCONV, "gbo_cc: no convergence");
+      break;
+    }
+  }
+  while (rc == GBO_OK) {  // make every tree a star
+    GBO_HIP_CHECK(hipMemsetAsync(changed, 0, sizeof(unsigned long long), st));
+    k_cc_jump<<<blocks_n, 256, 0, st>>>(f, n, changed);
+    GBO_LAUNCH_CHECK();
+    uint64_t h = 0;
+    rc = read_u64(changed, &h, st);
+    if (h == 0) break;
+  }
+  if (h_iters) *h_iters = it;
+  release_plan(&pv, st);
+  (void)hipFreeAsync(gf, st);
+  (void)hipFreeAsync(changed, st);
+  GBO_HIP_CHECK(hipStreamSynchronize(st));
+  return rc;
+}
+
+int gbo_tc(int bs, int32_t nrows, const int32_t* h_rp, const int32_t* h_ci, uint64_t* h_count, gbo_stream_t stream) {
+  GBO_TRY_BEGIN
+  if (!h_rp || !h_count || nrows < 0) return set_error(GBO_EINVAL, "gbo_tc: bad argument");
+  int64_t lnnz = 0;
+  int32_t *lp = nullptr, *li = nullptr, *tp = nullptr, *ti = nullptr;
+  int rc = gbo_csr_tril(nrows, h_rp, h_ci, 1, &lnnz, &lp, &li);
+  if (rc) return rc;
+  rc = gbo_csr_transpose(nrows, nrows, lp, li, &tp, &ti);
+  gbo_bsr L{}, LT{};
+  if (rc == GBO_OK) rc = gbo_bsr_create(bs, 0, lp, li, nrows, nrows, &L, stream);
+  if (rc == GBO_OK) rc = gbo_bsr_create(bs, 1, tp, ti, nrows, nrows, &LT, stream);
+  gbo_free_host(lp);
+  gbo_free_host(li);
+  gbo_free_host(tp);
+  gbo_free_host(ti);
+  uint64_t* d = nullptr;
+  if (rc == GBO_OK && hipMalloc((void**)&d, sizeof(uint64_t)) != hipSuccess) rc = set_error(GBO_ENOMEM, "gbo_tc: alloc");
+  if (rc == GBO_OK) rc = gbo_bsrbmm_tc(&L, &LT, d, stream);
+  if (rc == GBO_OK && hipMemcpy(h_count, d, sizeof(uint64_t), hipMemcpyDeviceToHost) != hipSuccess)
+    rc = set_error(GBO_EDEVICE, "gbo_tc: copy");
+  (void)hipFree(d);
+  gbo_bsr_free(&L);
+  gbo_bsr_free(&LT);
+  return rc;
+  GBO_TRY_END
+}
+
+}  // extern "C"

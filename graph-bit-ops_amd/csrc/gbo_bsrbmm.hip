@@ -1,0 +1,303 @@
+// bsrbmm: bit-BSR x bit-BSR products reduced to a scalar.
+//
+// Triangle counting (bmm{N}_sparse_masked_v5, bsrbmm/bsrbmm.cu:969,1214):
+//   count = sum_{i,k} A_ik (A B)_ik,  A = L packed by rows, B = L^T packed
+//   by columns (word k of B tile (J,K) = column k of that tile).
+// The reference walks, per mask tile (I,K), every tile (I,J) of block row I
+// and binary-searches B's block row J for K -- O(deg^2 log) per block row.
+// Here the block-column structure of B is transposed once (a stable counting
+// sort on the host, like the reference's csr2csc) so that the J's that
+// contribute to mask tile (I,K) are the intersection of two sorted lists:
+// block row I of A and block column K of B.  Each wave takes a fixed chunk
+// of mask tiles; its 64 lanes search the longer list for 64 entries of the
+// shorter one at a time, and for every match lane a (tile row) adds
+// popc(A_IJ[a] & B_JK[k]) for the columns k set in mask row a.
+//
+// Plain product sum (bmm{N}_sparse, bsrbmm/bsrbmm.cu:296-555):
+//   sum_{i,k} (A B)_ik = sum_j colsum_A(j) * rowsum_B(j),
+// two linear passes over the tiles instead of the reference's tile-pair loop.
+#include <algorithm>
+#include <vector>
+
+#include "gbo_device.hpp"
+#include "gbo_ops.hpp"
+
+namespace gbo {
+
+struct TcArgs {
+  const int32_t* a_rowptr;
+  const int32_t* a_colind;
+  const void* a_tiles;
+  int32_t nbr;
+  int64_t T;
+  const int32_t* plan;
+  int32_t chunk, nchunks;
+  const int32_t* bt_ptr;   // block-CSC of B: column K lists [bt_ptr[K], bt_ptr[K+1])
+  const int32_t* bt_row;   //   block row J of each entry (ascending inside a column)
+  const int32_t* bt_tile;  //   tile index in B of each entry
+  const void* b_tiles;
+  unsigned long long* count;
+};
+
+// First index in [lo, hi) of sorted arr whose value is >= key.
+__device__ __forceinline__ int64_t lower_bound_i32(const int32_t* __restrict__ arr, int64_t lo, int64_t hi, int32_t key) {
+  while (lo < hi) {
+    const int64_t mid = lo + ((hi - lo) >> 1);
+    if (arr[mid] < key) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+template <int BS>
+__global__ __launch_bounds__(256) void k_tc(const TcArgs a) {
+  using W = word_t<BS>;
+  const int lane = lane_id();
+  const int c = (int)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave);
+  if (c >= a.nchunks) return;
+  const int64_t cb = (int64_t)c * a.chunk;
+  const int64_t ce = min(cb + (int64_t)a.chunk, a.T);
+  const W* __restrict__ At = (const W*)a.a_tiles;
+  const W* __restrict__ Bt = (const W*)a.b_tiles;
+  uint64_t total = 0;
+
+  for (int32_t R0 = a.plan[c]; R0 < a.nbr;) {
+    const int32_t Rl = R0 + lane;
+    int32_t rsl = 0, rel = 0;
+    if (Rl < a.nbr) {
+      rsl = a.a_rowptr[Rl];
+      rel = a.a_rowptr[Rl + 1];
+    }
+    const bool belong = (Rl < a.nbr) && ((int64_t)rsl < ce);
+    const uint64_t bmask = __ballot(belong);
+    uint64_t work = __ballot(belong && rel > rsl);
+    while (work) {
+      const int j = __builtin_ctzll(work);
+      work &= work - 1;
+      const int64_t rs = __shfl(rsl, j), re = __shfl(rel, j);
+      const int64_t s = rs > cb ? rs : cb;
+      const int64_t te = re < ce ? re : ce;
+      for (int64_t tm = s; tm < te; ++tm) {  // mask tile (I, K)
+        const int32_t K = a.a_colind[tm];
+        const uint64_t M = (lane < BS) ? (uint64_t)At[tm * BS + lane] : 0ull;
+        // Union of the mask's column bits: the k's any row needs.
+        uint64_t un = M;
+        for (int off = 1; off < kWave; off <<= 1) un |= __shfl_xor(un, off);
+        if (un == 0) continue;
+        // Intersect block row I of A (J ascending) with block column K of B.
+        const int64_t la0 = rs, la1 = re;
+        const int64_t lb0 = a.bt_ptr[K], lb1 = a.bt_ptr[K + 1];
+        const bool a_short = (la1 - la0) <= (lb1 - lb0);
+        const int64_t s0 = a_short ? la0 : lb0, s1 = a_short ? la1 : lb1;
+        const int64_t l0 = a_short ? lb0 : la0, l1 = a_short ? lb1 : la1;
+        const int32_t* skeys = a_short ? a.a_colind : a.bt_row;
+        const int32_t* lkeys = a_short ? a.bt_row : a.a_colind;
+        for (int64_t base = s0; base < s1; base += kWave) {
+          const int64_t q = base + lane;
+          int64_t ta = -1, tb = -1;
+          if (q < s1) {
+            const int32_t J = skeys[q];
+            const int64_t pos = lower_bound_i32(lkeys, l0, l1, J);
+            if (pos < l1 && lkeys[pos] == J) {
+              ta = a_short ? q : pos;
+              tb = a_short ? a.bt_tile[pos] : a.bt_tile[q];
+            }
+          }
+          uint64_t hits = __ballot(ta >= 0);
+          while (hits) {
+            const int h = __builtin_ctzll(hits);
+            hits &= hits - 1;
+            const int64_t tA = __shfl(ta, h), tB = __shfl(tb, h);
+            const uint64_t X = (lane < BS) ? (uint64_t)At[tA * BS + lane] : 0ull;
+            const uint64_t Y = (lane < BS) ? (uint64_t)Bt[tB * BS + lane] : 0ull;
+            uint64_t u = un;
+            while (u) {
+              const int p = __builtin_ctzll(u);  // mask bit p <-> column k = BS-1-p
+              u &= u - 1;
+              const uint64_t yk = __shfl(Y, BS - 1 - p);
+              if ((M >> p) & 1ull) total += (uint64_t)popcw(X & yk);
+            }
+          }
+        }
+      }
+    }
+    if (__popcll(bmask) < kWave) break;
+    R0 += kWave;
+  }
+  for (int off = 1; off < kWave; off <<= 1) total += __shfl_xor(total, off);
+  if (lane == 0 && total) atomicAdd(a.count, (unsigned long long)total);
+}
+
+// colsum (rows=false): S[J*BS + c] += #rows of tile (I,J) with column c set.
+// rowsum of a column-direction B (rows=true): word k of tile (J,K) holds
+// column k with bit (BS-1-j) = row j, so S[J*BS + j] += #k with bit set --
+// the same per-bit count over the words of the tile, indexed by the tile's
+// block row instead of its block column.
+template <int BS>
+__global__ __launch_bounds__(256) void k_bitcount_by_col(const int32_t* __restrict__ rowptr,
+                                                         const int32_t* __restrict__ colind,
+                                                         const word_t<BS>* __restrict__ tiles, int64_t T,
+                                                         int32_t nbr, bool by_block_row,
+                                                         unsigned long long* __restrict__ S) {
+  const int lane = lane_id();
+  const int64_t t = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  if (t >= T) return;
+  const uint64_t w = (lane < BS) ? (uint64_t)tiles[t * BS + lane] : 0ull;
+  int32_t J;
+  if (by_block_row) {
+    // block row of tile t: upper_bound(rowptr, t) - 1
+    int32_t lo = 0, hi = nbr;
+    while (lo < hi) {
+      const int32_t mid = lo + (hi - lo) / 2;
+      if ((int64_t)rowptr[mid + 1] <= t) lo = mid + 1;
+      else hi = mid;
+    }
+    J = lo;
+  } else {
+    J = colind[t];
+  }
+  int mycnt = 0;
+  for (int cc = 0; cc < BS; ++cc) {
+    const uint64_t b = __ballot((w >> (BS - 1 - cc)) & 1ull);
+    if (lane == cc) mycnt = __popcll(b);
+  }
+  if (lane < BS && mycnt) atomicAdd(&S[(int64_t)J * BS + lane], (unsigned long long)mycnt);
+}
+
+__global__ __launch_bounds__(256) void k_dot_u64(const unsigned long long* __restrict__ x,
+                                                 const unsigned long long* __restrict__ y, int64_t n,
+                                                 unsigned long long* out) {
+  uint64_t s = 0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    s += x[i] * y[i];
+  for (int off = 1; off < kWave; off <<= 1) s += __shfl_xor(s, off);
+  if (lane_id() == 0 && s) atomicAdd(out, (unsigned long long)s);
+}
+
+}  // namespace gbo
+
+using namespace gbo;
+
+// Host-side stable transpose of a block structure (csr2csc on blocks).
+static int block_transpose(const gbo_bsr* B, int32_t** d_ptr, int32_t** d_row, int32_t** d_tile, hipStream_t st) {
+  const int32_t nbr = B->nblockrows, nbc = B->nblockcols;
+  const int64_t T = B->nblocks;
+  std::vector<int32_t> rp((size_t)nbr + 1), ci((size_t)T);
+  GBO_HIP_CHECK(hipMemcpyAsync(rp.data(), B->rowptr, sizeof(int32_t) * (nbr + 1), hipMemcpyDeviceToHost, st));
+  if (T) GBO_HIP_CHECK(hipMemcpyAsync(ci.data(), B->colind, sizeof(int32_t) * T, hipMemcpyDeviceToHost, st));
+  GBO_HIP_CHECK(hipStreamSynchronize(st));
+  std::vector<int32_t> cp((size_t)nbc + 1, 0), row((size_t)std::max<int64_t>(T, 1)), tile((size_t)std::max<int64_t>(T, 1));
+  for (int64_t t = 0; t < T; ++t) {
+    if (ci[t] < 0 || ci[t] >= nbc) return set_error(GBO_EINVAL, "block column index out of range");
+    cp[ci[t] + 1]++;
+  }
+  for (int32_t k = 0; k < nbc; ++k) cp[k + 1] += cp[k];
+  std::vector<int32_t> cur(cp.begin(), cp.end() - 1);
+  for (int32_t J = 0; J < nbr; ++J)
+    for (int32_t t = rp[J]; t < rp[J + 1]; ++t) {
+      const int32_t o = cur[ci[t]]++;
+      row[o] = J;
+      tile[o] = t;
+    }
+  GBO_HIP_CHECK(hipMallocAsync((void**)d_ptr, sizeof(int32_t) * ((size_t)nbc + 1), st));
+  GBO_HIP_CHECK(hipMallocAsync((void**)d_row, sizeof(int32_t) * row.size(), st));
+  GBO_HIP_CHECK(hipMallocAsync((void**)d_tile, sizeof(int32_t) * tile.size(), st));
+  GBO_HIP_CHECK(hipMemcpyAsync(*d_ptr, cp.data(), sizeof(int32_t) * cp.size(), hipMemcpyHostToDevice, st));
+  GBO_HIP_CHECK(hipMemcpyAsync(*d_row, row.data(), sizeof(int32_t) * row.size(), hipMemcpyHostToDevice, st));
+  GBO_HIP_CHECK(hipMemcpyAsync(*d_tile, tile.data(), sizeof(int32_t) * tile.size(), hipMemcpyHostToDevice, st));
+  GBO_HIP_CHECK(hipStreamSynchronize(st));  // host vectors die at return
+  return GBO_OK;
+}
+
+extern "C" {
+
+int gbo_bsrbmm_tc(const gbo_bsr* L, const gbo_bsr* LT, uint64_t* d_count, gbo_stream_t stream) {
+  GBO_TRY_BEGIN
+  int rc = validate_bsr(L, 0, "gbo_bsrbmm_tc(A)");
+  if (rc) return rc;
+  rc = validate_bsr(LT, 1, "gbo_bsrbmm_tc(B)");
+  if (rc) return rc;
+  if (L->bs != LT->bs) return set_error(GBO_EINVAL, "gbo_bsrbmm_tc: block sizes differ");
+  if (L->nrows != L->ncols) return set_error(GBO_EINVAL, "gbo_bsrbmm_tc: the mask A must be square");
+  if (L->ncols != LT->nrows || L->nrows != LT->ncols)
+    return set_error(GBO_EINVAL, "gbo_bsrbmm_tc: B must be %d x %d", L->ncols, L->nrows);
+  if (!d_count) return set_error(GBO_EINVAL, "gbo_bsrbmm_tc: null count");
+  hipStream_t st = stream;
+  GBO_HIP_CHECK(hipMemsetAsync(d_count, 0, sizeof(uint64_t), st));
+  if (L->nblocks == 0 || LT->nblocks == 0) return GBO_OK;
+  int32_t *bt_ptr = nullptr, *bt_row = nullptr, *bt_tile = nullptr;
+  rc = block_transpose(LT, &bt_ptr, &bt_row, &bt_tile, st);
+  if (rc) return rc;
+  // TC work per mask tile is far larger than a bsrbmv tile: small chunks.
+  gbo_bsr tmp = *L;
+  tmp.plan = nullptr;
+  tmp.plan_chunk = 0;
+  rc = gbo_bsr_plan(&tmp, 16, st);
+  if (rc == GBO_OK) {
+    TcArgs a{};
+    a.a_rowptr = L->rowptr;
+    a.a_colind = L->colind;
+    a.a_tiles = L->tiles;
+    a.nbr = L->nblockrows;
+    a.T = L->nblocks;
+    a.plan = tmp.plan;
+    a.chunk = tmp.plan_chunk;
+    a.nchunks = tmp.plan_nchunks;
+    a.bt_ptr = bt_ptr;
+    a.bt_row = bt_row;
+    a.bt_tile = bt_tile;
+    a.b_tiles = LT->tiles;
+    a.count = (unsigned long long*)d_count;
+    const unsigned blocks = (unsigned)(((int64_t)a.nchunks + 3) / 4);
+    switch (L->bs) {
+      case 8: k_tc<8><<<blocks, 256, 0, st>>>(a); break;
+      case 16: k_tc<16><<<blocks, 256, 0, st>>>(a); break;
+      case 32: k_tc<32><<<blocks, 256, 0, st>>>(a); break;
+      case 64: k_tc<64><<<blocks, 256, 0, st>>>(a); break;
+    }
+    if (hipGetLastError() != hipSuccess) rc = set_error(GBO_EDEVICE, "k_tc launch failed");
+  }
+  (void)hipStreamSynchronize(st);
+  gbo_bsr_plan_free(&tmp);
+  (void)hipFreeAsync(bt_ptr, st);
+  (void)hipFreeAsync(bt_row, st);
+  (void)hipFreeAsync(bt_tile, st);
+  return rc;
+  GBO_TRY_END
+}
+
+int gbo_bsrbmm_sum(const gbo_bsr* A, const gbo_bsr* B, uint64_t* d_sum, gbo_stream_t stream) {
+  int rc = validate_bsr(A, 0, "gbo_bsrbmm_sum(A)");
+  if (rc) return rc;
+  rc = validate_bsr(B, 1, "gbo_bsrbmm_sum(B)");
+  if (rc) return rc;
+  if (A->bs != B->bs) return set_error(GBO_EINVAL, "gbo_bsrbmm_sum: block sizes differ");
+  if (A->ncols != B->nrows) return set_error(GBO_EINVAL, "gbo_bsrbmm_sum: inner dimensions differ");
+  if (!d_sum) return set_error(GBO_EINVAL, "gbo_bsrbmm_sum: null output");
+  hipStream_t st = stream;
+  const int bs = A->bs;
+  const int64_t n = (int64_t)A->nblockcols * bs;
+  unsigned long long* S = nullptr;
+  GBO_HIP_CHECK(hipMallocAsync((void**)&S, sizeof(unsigned long long) * 2 * std::max<int64_t>(n, 1), st));
+  GBO_HIP_CHECK(hipMemsetAsync(S, 0, sizeof(unsigned long long) * 2 * std::max<int64_t>(n, 1), st));
+  GBO_HIP_CHECK(hipMemsetAsync(d_sum, 0, sizeof(uint64_t), st));
+  auto launch = [&](const gbo_bsr* M, bool by_row, unsigned long long* out) {
+    if (M->nblocks == 0) return;
+    const unsigned blocks = (unsigned)((M->nblocks + 3) / 4);
+    switch (bs) {
+      case 8: k_bitcount_by_col<8><<<blocks, 256, 0, st>>>(M->rowptr, M->colind, (const uint8_t*)M->tiles, M->nblocks, M->nblockrows, by_row, out); break;
+      case 16: k_bitcount_by_col<16><<<blocks, 256, 0, st>>>(M->rowptr, M->colind, (const uint16_t*)M->tiles, M->nblocks, M->nblockrows, by_row, out); break;
+      case 32: k_bitcount_by_col<32><<<blocks, 256, 0, st>>>(M->rowptr, M->colind, (const uint32_t*)M->tiles, M->nblocks, M->nblockrows, by_row, out); break;
+      case 64: k_bitcount_by_col<64><<<blocks, 256, 0, st>>>(M->rowptr, M->colind, (const uint64_t*)M->tiles, M->nblocks, M->nblockrows, by_row, out); break;
+    }
+  };
+  launch(A, false, S);
+  launch(B, true, S + n);
+  GBO_LAUNCH_CHECK();
+  k_dot_u64<<<256, 256, 0, st>>>(S, S + n, n, (unsigned long long*)d_sum);
+  GBO_LAUNCH_CHECK();
+  GBO_HIP_CHECK(hipFreeAsync(S, st));
+  return GBO_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,81 @@
+// Device-side helpers for gfx950 (CDNA4, 64-lane wavefronts).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <climits>
+#include <cstdint>
+
+#include "gbo_internal.hpp"
+
+namespace gbo {
+
+constexpr int kWave = 64;
+
+template <int BS> struct WordOf;
+template <> struct WordOf<8> { using type = uint8_t; };
+template <> struct WordOf<16> { using type = uint16_t; };
+template <> struct WordOf<32> { using type = uint32_t; };
+template <> struct WordOf<64> { using type = uint64_t; };
+template <int BS> using word_t = typename WordOf<BS>::type;
+
+__device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & (kWave - 1)); }
+
+__device__ __forceinline__ int popcw(uint64_t x) { return __popcll(x); }
+__device__ __forceinline__ int popcw(uint32_t x) { return __popc(x); }
+__device__ __forceinline__ int popcw(uint16_t x) { return __popc((uint32_t)x); }
+__device__ __forceinline__ int popcw(uint8_t x) { return __popc((uint32_t)x); }
+
+// Reverses the low BS bits of x (bit 0 <-> bit BS-1).
+template <int BS>
+__device__ __forceinline__ uint64_t brev_low(uint64_t x) {
+  return __brevll(x) >> (64 - BS);
+}
+
+// Mask with the low BS bits set.
+template <int BS>
+__device__ __forceinline__ uint64_t low_mask() {
+  if constexpr (BS == 64) return ~0ull;
+  else return (1ull << BS) - 1;
+}
+
+// Atomic OR of a BS-bit word into a word array (bytes and shorts go through
+// the aligned 32-bit word that contains them; little-endian).  Returns the
+// previous value of the BS-bit word.
+template <int BS>
+__device__ __forceinline__ uint64_t atomic_or_word(word_t<BS>* base, int64_t idx, uint64_t v) {
+  if constexpr (BS == 64) {
+    return atomicOr((unsigned long long*)(base + idx), (unsigned long long)v);
+  } else if constexpr (BS == 32) {
+    return atomicOr((unsigned int*)(base + idx), (unsigned int)v);
+  } else {
+    const uintptr_t addr = (uintptr_t)(base + idx);
+    unsigned int* w32 = (unsigned int*)(addr & ~(uintptr_t)3);
+    const int shift = (int)(addr & 3) * 8;
+    const unsigned int old = atomicOr(w32, (unsigned int)(v << shift));
+    return (old >> shift) & (unsigned int)low_mask<BS>();
+  }
+}
+
+// Reductions over the lanes of one group of BS lanes, or across the G
+// groups that hold the same row index (xor offsets BS, 2BS, ...).
+template <int BS, class T>
+__device__ __forceinline__ T group_min(T v) {
+  for (int off = 1; off < BS; off <<= 1) {
+    T o = __shfl_xor(v, off);
+    v = o < v ? o : v;
+  }
+  return v;
+}
+
+}  // namespace gbo
+
+#define GBO_HIP_CHECK(expr)                                                              \
+  do {                                                                                   \
+    hipError_t _e = (expr);                                                              \
+    if (_e != hipSuccess)                                                                \
+      return gbo::set_error(GBO_EDEVICE, "%s:%d %s: %s", __FILE__, __LINE__, #expr,      \
+                            hipGetErrorString(_e));                                      \
+  } while (0)
+
+#define GBO_LAUNCH_CHECK() GBO_HIP_CHECK(hipGetLastError())

@@ -1,0 +1,58 @@
+// Shared declarations between the packer, the bsrbmv/bsrbmm operations and
+// the drivers (device-side translation units).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "gbo_device.hpp"
+
+namespace gbo {
+
+// Work decomposition of a bit-BSR: the tiles [0, nblocks) are cut into
+// chunks of `chunk` consecutive tiles; plan[c] is the first block row that
+// chunk c touches (a row that begins before the chunk is "split" and its
+// partial results are combined across chunks).
+struct PlanView {
+  const int32_t* plan = nullptr;
+  int32_t chunk = 0;
+  int32_t nchunks = 0;
+  bool owned = false;
+};
+
+int default_chunk(int bs);
+int acquire_plan(const gbo_bsr* A, PlanView* pv, hipStream_t st);
+int release_plan(PlanView* pv, hipStream_t st);
+int validate_bsr(const gbo_bsr* A, int want_dir, const char* who);
+
+template <class TI, class TO>
+int scan_exclusive(const TI* in, TO* out, int64_t n, hipStream_t st);
+
+constexpr uint32_t kInfU32 = 0xFFFFFFFFu;
+
+// bsrbmv operation codes.
+enum SpmvOp { OP_COUNT = 0, OP_BFS = 1, OP_MINPLUS = 2, OP_SUM = 3, OP_MIN = 4 };
+
+struct SpmvArgs {
+  const int32_t* rowptr;
+  const int32_t* colind;
+  const void* tiles;
+  int32_t nrows, ncols, nbr;
+  int64_t T;
+  const int32_t* plan;
+  int32_t chunk, nchunks;
+  const void* xbits;    // COUNT: x bits; BFS: frontier bits
+  const void* visited;  // BFS: visited bits (per block row)
+  void* out;            // COUNT int32 y / BFS next bits / MINPLUS uint32 / SUM float / MIN int32
+  const uint32_t* w;    // MINPLUS weights in tile bit order (nullable)
+  const int64_t* woff;  // MINPLUS per-tile weight offsets
+  const void* xval;     // MINPLUS dist / SUM x / MIN x
+  int64_t* count;       // BFS: number of newly set bits
+  float* partial;       // SUM: split-row partials [nchunks][2][bs]
+  const uint64_t* active;  // MINPLUS: optional per-block-column activity bits (nullable)
+  int accumulate;       // COUNT: add into y
+  float scale, bias;    // SUM: y = scale * (A x) + bias  (scale 1, bias 0 for the plain op)
+};
+
+int spmv_launch(int bs, int op, const SpmvArgs& a, hipStream_t st);
+
+}  // namespace gbo

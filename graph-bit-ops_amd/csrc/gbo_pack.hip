@@ -1,0 +1,598 @@
+// CSR -> bit-BSR packing on the GPU (replaces csr2bsr_batch_{N}, algorithm/
+// csr2bsr_batch_bsrbmv.cu:180-947, and csr2bsr_batch_{N}_{row,col},
+// bsrbmm/csr2bsr_batch.cu), vector packing (ToBit{N}Row) and the work
+// decomposition ("plan") used by every bsrbmv/bsrbmm operation.
+//
+// The reference packs one block row per host-loop iteration through dense
+// float tiles (cusparseScsr2bsr) and a ballot kernel.  Here one wavefront
+// packs 64/bs block rows at once: lane r of a group follows CSR row r of its
+// block row, the group takes the minimum pending block column (a k-way merge
+// of the bs sorted rows), and every lane ORs its entries of that block
+// column into its own row word -- which is directly word r of the tile in
+// the reference layout (MSB = column 0).  Two passes: count tiles per block
+// row, scan, then fill colind/tiles.  No dense staging, no host round trip.
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "gbo_device.hpp"
+#include "gbo_ops.hpp"
+
+namespace gbo {
+
+// ------------------------------------------------------------------ scan
+constexpr int kScanThreads = 256;
+constexpr int kScanPer = 8;
+constexpr int kScanTile = kScanThreads * kScanPer;
+
+template <class TI>
+__global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const TI* __restrict__ in, int64_t n,
+                                                              int64_t* __restrict__ bsum) {
+  __shared__ int64_t sh[kScanThreads];
+  const int64_t base = (int64_t)blockIdx.x * kScanTile;
+  int64_t s = 0;
+  for (int k = 0; k < kScanPer; ++k) {
+    const int64_t i = base + threadIdx.x + (int64_t)k * kScanThreads;
+    if (i < n) s += (int64_t)in[i];
+  }
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int off = kScanThreads / 2; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) bsum[blockIdx.x] = sh[0];
+}
+
+// Exclusive scan of one tile; thread t owns elements [base+t*P, base+t*P+P).
+// In-place safe: each element is read and written by the same thread.
+template <class TI, class TO>
+__global__ __launch_bounds__(kScanThreads) void k_scan_apply(const TI* in, int64_t n,
+                                                             const int64_t* __restrict__ boff,
+                                                             TO* out) {
+  __shared__ int64_t sh[kScanThreads];
+  const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanPer;
+  int64_t v[kScanPer];
+  int64_t s = 0;
+  for (int k = 0; k < kScanPer; ++k) {
+    const int64_t i = base + k;
+    v[k] = (i < n) ? (int64_t)in[i] : 0;
+    s += v[k];
+  }
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int off = 1; off < kScanThreads; off <<= 1) {
+    const int64_t o = ((int)threadIdx.x >= off) ? sh[threadIdx.x - off] : 0;
+    __syncthreads();
+    sh[threadIdx.x] += o;
+    __syncthreads();
+  }
+  int64_t run = sh[threadIdx.x] - s + (boff ? boff[blockIdx.x] : 0);
+  for (int k = 0; k < kScanPer; ++k) {
+    const int64_t i = base + k;
+    if (i < n) out[i] = (TO)run;
+    run += v[k];
+    if (i == n - 1) out[n] = (TO)run;
+  }
+}
+
+// out[0..n] = exclusive prefix sums of in[0..n-1]; out[n] = total.
+template <class TI, class TO>
+int scan_exclusive(const TI* in, TO* out, int64_t n, hipStream_t st) {
+  if (n <= 0) {
+    GBO_HIP_CHECK(hipMemsetAsync(out, 0, sizeof(TO), st));
+    return GBO_OK;
+  }
+  const int64_t nb = (n + kScanTile - 1) / kScanTile;
+  if (nb == 1) {
+    k_scan_apply<TI, TO><<<1, kScanThreads, 0, st>>>(in, n, nullptr, out);
+    GBO_LAUNCH_CHECK();
+    return GBO_OK;
+  }
+  int64_t* bsum = nullptr;
+  GBO_HIP_CHECK(hipMallocAsync((void**)&bsum, sizeof(int64_t) * (2 * nb + 1), st));
+  int64_t* boff = bsum + nb;
+  k_scan_reduce<TI><<<(unsigned)nb, kScanThreads, 0, st>>>(in, n, bsum);
+  GBO_LAUNCH_CHECK();
+  int rc = scan_exclusive<int64_t, int64_t>(bsum, boff, nb, st);
+  if (rc == GBO_OK) {
+    k_scan_apply<TI, TO><<<(unsigned)nb, kScanThreads, 0, st>>>(in, n, boff, out);
+    GBO_LAUNCH_CHECK();
+  }
+  GBO_HIP_CHECK(hipFreeAsync(bsum, st));
+  return rc;
+}
+
+template int scan_exclusive<int32_t, int32_t>(const int32_t*, int32_t*, int64_t, hipStream_t);
+template int scan_exclusive<int32_t, int64_t>(const int32_t*, int64_t*, int64_t, hipStream_t);
+template int scan_exclusive<int64_t, int64_t>(const int64_t*, int64_t*, int64_t, hipStream_t);
+
+// ---------------------------------------------------------------- packing
+enum { PACK_COUNT = 0, PACK_ROW = 1, PACK_COL = 2, PACK_VALUES = 3 };
+
+// Error flags raised by the packer (device int, OR-ed).
+enum { PACKERR_COLRANGE = 1, PACKERR_UNSORTED = 2 };
+
+template <int BS, int MODE>
+__global__ __launch_bounds__(256) void k_pack(const int32_t* __restrict__ rp,
+                                              const int32_t* __restrict__ ci, int32_t nrows,
+                                              int32_t ncols, int32_t nbr,
+                                              int32_t* __restrict__ bsr_rp,  // counts (COUNT) or rowptr
+                                              int32_t* __restrict__ out_col,
+                                              word_t<BS>* __restrict__ out_tiles,
+                                              const uint32_t* __restrict__ val,
+                                              const int64_t* __restrict__ woff,
+                                              uint32_t* __restrict__ tval, int* __restrict__ err) {
+  using W = word_t<BS>;
+  constexpr int G = kWave / BS;
+  const int lane = lane_id();
+  const int g = lane / BS, r = lane % BS;
+  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const int64_t I = wave * G + g;
+  if (wave * G >= nbr) return;  // wave-uniform
+  const bool blk_ok = I < nbr;
+  const int64_t row = I * BS + r;
+  int32_t p = 0, pe = 0;
+  if (blk_ok && row < nrows) {
+    p = rp[row];
+    pe = rp[row + 1];
+  }
+  auto blockcol = [&](int32_t q) -> int32_t {
+    const int32_t c = ci[q];
+    if (MODE == PACK_COUNT && (c < 0 || c >= ncols)) {
+      atomicOr(err, PACKERR_COLRANGE);
+      return INT_MAX - 1;  // poison: still terminates
+    }
+    return c / BS;
+  };
+  int32_t cur = (p < pe) ? blockcol(p) : INT_MAX;
+  int64_t base = 0;
+  if (MODE != PACK_COUNT && blk_ok) base = bsr_rp[I];
+  int32_t k = 0;
+  while (true) {
+    const int32_t m = group_min<BS>(cur);
+    if (__all(m == INT_MAX)) break;
+    uint64_t w = 0;
+    int32_t p0 = p;
+    if (m != INT_MAX) {
+      while (cur == m) {
+        w |= 1ull << (BS - 1 - (ci[p] % BS));
+        ++p;
+        const int32_t nxt = (p < pe) ? blockcol(p) : INT_MAX;
+        if (MODE == PACK_COUNT && nxt < cur) atomicOr(err, PACKERR_UNSORTED);
+        cur = nxt;
+      }
+    }
+    if constexpr (MODE == PACK_ROW) {
+      if (m != INT_MAX) {
+        out_tiles[(base + k) * BS + r] = (W)w;
+        if (r == 0) out_col[base + k] = m;
+      }
+    } else if constexpr (MODE == PACK_COL) {
+      // Transpose the group's bs x bs bit tile with one ballot per column:
+      // ballot bit (g*BS + r) = row r has column kk.
+      uint64_t mine = 0;
+      for (int kk = 0; kk < BS; ++kk) {
+        const uint64_t b = __ballot((w >> (BS - 1 - kk)) & 1ull);
+        const uint64_t grp = (b >> (g * BS)) & low_mask<BS>();
+        if (kk == r) mine = brev_low<BS>(grp);
+      }
+      if (m != INT_MAX) {
+        out_tiles[(base + k) * BS + r] = (W)mine;
+        if (r == 0) out_col[base + k] = m;
+      }
+    } else if constexpr (MODE == PACK_VALUES) {
+      // Rank of this row's first bit inside the tile = popcounts of the
+      // rows above it (exclusive prefix over the group).
+      int32_t cnt = (m != INT_MAX) ? popcw(w) : 0;
+      int32_t incl = cnt;
+      for (int off = 1; off < BS; off <<= 1) {
+        const int32_t o = __shfl_up(incl, off, BS);
+        if (r >= off) incl += o;
+      }
+      if (m != INT_MAX) {
+        int64_t pos = woff[base + k] + (incl - cnt);
+        for (int32_t q = p0; q < p; ++q) {
+          if (q > p0 && ci[q] == ci[q - 1]) {
+            tval[pos - 1] = min(tval[pos - 1], val[q]);
+          } else {
+            tval[pos++] = val[q];
+          }
+        }
+      }
+    }
+    if (m != INT_MAX) ++k;
+  }
+  if constexpr (MODE == PACK_COUNT) {
+    if (blk_ok && r == 0) bsr_rp[I] = k;
+  }
+}
+
+template <int BS>
+__global__ __launch_bounds__(256) void k_tile_popc(const word_t<BS>* __restrict__ tiles, int64_t T,
+                                                   int64_t* __restrict__ cnt) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= T) return;
+  int64_t s = 0;
+  for (int r = 0; r < BS; ++r) s += popcw(tiles[t * BS + r]);
+  cnt[t] = s;
+}
+
+template <int BS>
+__global__ __launch_bounds__(256) void k_pack_vec(const float* __restrict__ x, word_t<BS>* __restrict__ xb,
+                                                  int32_t n, int32_t nw) {
+  const int32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nw) return;
+  uint64_t w = 0;
+  for (int k = 0; k < BS; ++k) {
+    const int64_t i = (int64_t)j * BS + k;
+    if (i < n && x[i] > 0.0f) w |= 1ull << (BS - 1 - k);
+  }
+  xb[j] = (word_t<BS>)w;
+}
+
+// First block row of every chunk of `chunk` tiles: the first row R with
+// rowptr[R] >= b, or the row containing tile b when that row starts before b.
+__global__ __launch_bounds__(256) void k_plan(const int32_t* __restrict__ rowptr, int32_t nbr,
+                                              int32_t chunk, int32_t nchunks,
+                                              int32_t* __restrict__ plan) {
+  const int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nchunks) return;
+  const int64_t b = (int64_t)c * chunk;
+  int32_t lo = 0, hi = nbr;  // search in rowptr[0..nbr]
+  while (lo < hi) {
+    const int32_t mid = lo + (hi - lo) / 2;
+    if ((int64_t)rowptr[mid] < b) lo = mid + 1;
+    else hi = mid;
+  }
+  int32_t R = lo;
+  if (R > nbr || (int64_t)rowptr[R] > b) R -= 1;
+  if (R >= nbr) R = nbr > 0 ? nbr - 1 : 0;
+  if (R < 0) R = 0;
+  plan[c] = R;
+}
+
+int default_chunk(int bs) {
+  switch (bs) {
+    case 8: return 4096;
+    case 16: return 2048;
+    case 32: return 512;
+    default: return 256;
+  }
+}
+
+static bool bs_ok(int bs) { return bs == 8 || bs == 16 || bs == 32 || bs == 64; }
+
+template <int BS, int MODE>
+static int launch_pack(const int32_t* rp, const int32_t* ci, int32_t nrows, int32_t ncols, int32_t nbr,
+                       int32_t* bsr_rp, int32_t* out_col, void* out_tiles, const uint32_t* val,
+                       const int64_t* woff, uint32_t* tval, int* err, hipStream_t st) {
+  constexpr int G = kWave / BS;
+  const int64_t waves = ((int64_t)nbr + G - 1) / G;
+  const int64_t blocks = (waves + 3) / 4;
+  if (blocks == 0) return GBO_OK;
+  k_pack<BS, MODE><<<(unsigned)blocks, 256, 0, st>>>(rp, ci, nrows, ncols, nbr, bsr_rp, out_col,
+                                                     (word_t<BS>*)out_tiles, val, woff, tval, err);
+  GBO_LAUNCH_CHECK();
+  return GBO_OK;
+}
+
+template <int MODE>
+static int dispatch_pack(int bs, const int32_t* rp, const int32_t* ci, int32_t nrows, int32_t ncols,
+                         int32_t nbr, int32_t* bsr_rp, int32_t* out_col, void* out_tiles,
+                         const uint32_t* val, const int64_t* woff, uint32_t* tval, int* err,
+                         hipStream_t st) {
+  switch (bs) {
+    case 8: return launch_pack<8, MODE>(rp, ci, nrows, ncols, nbr, bsr_rp, out_col, out_tiles, val, woff, tval, err, st);
+    case 16: return launch_pack<16, MODE>(rp, ci, nrows, ncols, nbr, bsr_rp, out_col, out_tiles, val, woff, tval, err, st);
+    case 32: return launch_pack<32, MODE>(rp, ci, nrows, ncols, nbr, bsr_rp, out_col, out_tiles, val, woff, tval, err, st);
+    case 64: return launch_pack<64, MODE>(rp, ci, nrows, ncols, nbr, bsr_rp, out_col, out_tiles, val, woff, tval, err, st);
+  }
+  return set_error(GBO_EUNSUPPORTED, "block size %d not supported (8, 16, 32, 64)", bs);
+}
+
+// Plan for a matrix: uses A->plan when present, else builds a temporary one
+// (stream-ordered allocation, released by release_plan).
+int acquire_plan(const gbo_bsr* A, PlanView* pv, hipStream_t st) {
+  if (A->plan && A->plan_chunk > 0 && A->plan_nchunks > 0) {
+    pv->plan = A->plan;
+    pv->chunk = A->plan_chunk;
+    pv->nchunks = A->plan_nchunks;
+    pv->owned = false;
+    return GBO_OK;
+  }
+  const int32_t chunk = default_chunk(A->bs);
+  const int64_t nch = std::max<int64_t>(1, (A->nblocks + chunk - 1) / chunk);
+  if (nch >= INT_MAX) return set_error(GBO_ERANGE, "too many chunks");
+  int32_t* plan = nullptr;
+  GBO_HIP_CHECK(hipMallocAsync((void**)&plan, sizeof(int32_t) * nch, st));
+  k_plan<<<(unsigned)((nch + 255) / 256), 256, 0, st>>>(A->rowptr, A->nblockrows, chunk, (int32_t)nch, plan);
+  GBO_LAUNCH_CHECK();
+  pv->plan = plan;
+  pv->chunk = chunk;
+  pv->nchunks = (int32_t)nch;
+  pv->owned = true;
+  return GBO_OK;
+}
+
+int release_plan(PlanView* pv, hipStream_t st) {
+  if (pv->owned && pv->plan) GBO_HIP_CHECK(hipFreeAsync(const_cast<int32_t*>(pv->plan), st));
+  pv->plan = nullptr;
+  pv->owned = false;
+  return GBO_OK;
+}
+
+int validate_bsr(const gbo_bsr* A, int want_dir, const char* who) {
+  if (!A) return set_error(GBO_EINVAL, "%s: null matrix", who);
+  if (!bs_ok(A->bs)) return set_error(GBO_EUNSUPPORTED, "%s: block size %d not supported", who, A->bs);
+  if (want_dir >= 0 && A->dir != want_dir)
+    return set_error(GBO_EINVAL, "%s: matrix must have dir %d", who, want_dir);
+  if (A->nrows < 0 || A->ncols < 0 || A->nblocks < 0)
+    return set_error(GBO_EINVAL, "%s: negative size", who);
+  if (A->nblockrows != (A->nrows + A->bs - 1) / A->bs || A->nblockcols != (A->ncols + A->bs - 1) / A->bs)
+    return set_error(GBO_EINVAL, "%s: nblockrows/nblockcols inconsistent with nrows/ncols", who);
+  if (A->nblocks >= ((int64_t)1 << 31)) return set_error(GBO_ERANGE, "%s: nblocks exceeds int32", who);
+  if (!A->rowptr || (A->nblocks > 0 && (!A->colind || !A->tiles)))
+    return set_error(GBO_EINVAL, "%s: null device array", who);
+  return GBO_OK;
+}
+
+}  // namespace gbo
+
+using namespace gbo;
+
+static int check_err_flag(int* d_err, hipStream_t st, const char* who) {
+  int h = 0;
+  GBO_HIP_CHECK(hipMemcpyAsync(&h, d_err, sizeof(int), hipMemcpyDeviceToHost, st));
+  GBO_HIP_CHECK(hipStreamSynchronize(st));
+  if (h & PACKERR_COLRANGE) return set_error(GBO_EINVAL, "%s: column index out of range", who);
+  if (h & PACKERR_UNSORTED) return set_error(GBO_EINVAL, "%s: CSR rows must be sorted ascending", who);
+  return GBO_OK;
+}
+
+extern "C" {
+
+int gbo_csr2bsr_nnzb_dev(int bs, const int32_t* d_rp, const int32_t* d_ci, int32_t nrows, int32_t ncols,
+                         int32_t* d_bsrRowPtr, int64_t* nblocks, gbo_stream_t stream) {
+  if (!bs_ok(bs)) return set_error(GBO_EUNSUPPORTED, "block size %d not supported (8, 16, 32, 64)", bs);
+  if (nrows < 0 || ncols < 0 || !d_rp || !d_bsrRowPtr || !nblocks)
+    return set_error(GBO_EINVAL, "gbo_csr2bsr_nnzb_dev: bad argument");
+  hipStream_t st = stream;
+  const int32_t nbr = (nrows + bs - 1) / bs;
+  int* d_err = nullptr;
+  GBO_HIP_CHECK(hipMallocAsync((void**)&d_err, sizeof(int), st));
+  GBO_HIP_CHECK(hipMemsetAsync(d_err, 0, sizeof(int), st));
+  int rc = dispatch_pack<PACK_COUNT>(bs, d_rp, d_ci, nrows, ncols, nbr, d_bsrRowPtr, nullptr, nullptr,
+                                     nullptr, nullptr, nullptr, d_err, st);
+  if (rc == GBO_OK) rc = check_err_flag(d_err, st, "gbo_csr2bsr_nnzb_dev");
+  (void)hipFreeAsync(d_err, st);
+  if (rc != GBO_OK) return rc;
+  // counts -> exclusive scan (in place); total must fit int32.
+  int64_t* tmp = nullptr;
+  GBO_HIP_CHECK(hipMallocAsync((void**)&tmp, sizeof(int64_t) * ((size_t)nbr + 1), st));
+  rc = scan_exclusive<int32_t, int64_t>(d_bsrRowPtr, tmp, nbr, st);
+  int64_t total = 0;
+  if (rc == GBO_OK) {
+    GBO_HIP_CHECK(hipMemcpyAsync(&total, tmp + nbr, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    GBO_HIP_CHECK(hipStreamSynchronize(st));
+    if (total >= ((int64_t)1 << 31)) rc = set_error(GBO_ERANGE, "tile count %lld exceeds int32", (long long)total);
+  }
+  if (rc == GBO_OK) rc = scan_exclusive<int32_t, int32_t>(d_bsrRowPtr, d_bsrRowPtr, nbr, st);
+  (void)hipFreeAsync(tmp, st);
+  if (rc != GBO_OK) return rc;
+  GBO_HIP_CHECK(hipStreamSynchronize(st));
+  *nblocks = total;
+  return GBO_OK;
+}
+
+int gbo_csr2bsr_fill_dev(int bs, int dir, const int32_t* d_rp, const int32_t* d_ci, int32_t nrows,
+                         int32_t ncols, const int32_t* d_bsrRowPtr, int32_t* d_bsrColInd, void* d_tiles,
+                         gbo_stream_t stream) {
+  if (!bs_ok(bs)) return set_error(GBO_EUNSUPPORTED, "block size %d not supported (8, 16, 32, 64)", bs);
+  if (dir != 0 && dir != 1) return set_error(GBO_EINVAL, "dir must be 0 (row) or 1 (col)");
+  if (nrows < 0 || ncols < 0 || !d_rp || !d_bsrRowPtr) return set_error(GBO_EINVAL, "gbo_csr2bsr_fill_dev: bad argument");
+  const int32_t nbr = (nrows + bs - 1) / bs;
+  if (dir == 0)
+    return dispatch_pack<PACK_ROW>(bs, d_rp, d_ci, nrows, ncols, nbr, const_cast<int32_t*>(d_bsrRowPtr),
+                                   d_bsrColInd, d_tiles, nullptr, nullptr, nullptr, nullptr, stream);
+  return dispatch_pack<PACK_COL>(bs, d_rp, d_ci, nrows, ncols, nbr, const_cast<int32_t*>(d_bsrRowPtr),
+                                 d_bsrColInd, d_tiles, nullptr, nullptr, nullptr, nullptr, stream);
+}
+
+int gbo_bsr_create_dev(int bs, int dir, const int32_t* d_rp, const int32_t* d_ci, int32_t nrows,
+                       int32_t ncols, gbo_bsr* out, gbo_stream_t stream) {
+  if (!out) return set_error(GBO_EINVAL, "gbo_bsr_create_dev: null output");
+  std::memset(out, 0, sizeof(*out));
+  if (!bs_ok(bs)) return set_error(GBO_EUNSUPPORTED, "block size %d not supported (8, 16, 32, 64)", bs);
+  if (dir != 0 && dir != 1) return set_error(GBO_EINVAL, "dir must be 0 (row) or 1 (col)");
+  const int32_t nbr = (nrows + bs - 1) / bs;
+  gbo_bsr A{};
+  A.bs = bs;
+  A.dir = dir;
+  A.nrows = nrows;
+  A.ncols = ncols;
+  A.nblockrows = nbr;
+  A.nblockcols = (ncols + bs - 1) / bs;
+  GBO_HIP_CHECK(hipMalloc((void**)&A.rowptr, sizeof(int32_t) * ((size_t)nbr + 1)));
+  int rc = gbo_csr2bsr_nnzb_dev(bs, d_rp, d_ci, nrows, ncols, A.rowptr, &A.nblocks, stream);
+  if (rc != GBO_OK) {
+    (void)hipFree(A.rowptr);
+    return rc;
+  }
+  const size_t tile_bytes = (size_t)A.nblocks * bs * (bs / 8);
+  hipError_t e1 = hipMalloc((void**)&A.colind, sizeof(int32_t) * std::max<int64_t>(A.nblocks, 1));
+  hipError_t e2 = hipMalloc(&A.tiles, std::max<size_t>(tile_bytes, 8));
+  if (e1 != hipSuccess || e2 != hipSuccess) {
+    gbo_bsr_free(&A);
+    return set_error(GBO_ENOMEM, "gbo_bsr_create: cannot allocate %zu tile bytes", tile_bytes);
+  }
+  rc = gbo_csr2bsr_fill_dev(bs, dir, d_rp, d_ci, nrows, ncols, A.rowptr, A.colind, A.tiles, stream);
+  if (rc != GBO_OK) {
+    gbo_bsr_free(&A);
+    return rc;
+  }
+  *out = A;
+  return GBO_OK;
+}
+
+// Uploads a host CSR, sorting a copy of any unsorted row first.
+static int upload_csr(const int32_t* h_rp, const int32_t* h_ci, int32_t nrows, int32_t ncols,
+                      int32_t** d_rp, int32_t** d_ci) {
+  if (nrows < 0 || ncols < 0 || !h_rp) return set_error(GBO_EINVAL, "bad CSR");
+  if (h_rp[0] != 0) return set_error(GBO_EINVAL, "rowptr[0] must be 0");
+  for (int32_t r = 0; r < nrows; ++r)
+    if (h_rp[r + 1] < h_rp[r]) return set_error(GBO_EINVAL, "rowptr not monotone at row %d", r);
+  const int64_t nnz = h_rp[nrows];
+  if (nnz > 0 && !h_ci) return set_error(GBO_EINVAL, "null colind");
+  bool sorted = true;
+  for (int32_t r = 0; r < nrows && sorted; ++r)
+    for (int32_t k = h_rp[r] + 1; k < h_rp[r + 1]; ++k)
+      if (h_ci[k] < h_ci[k - 1]) {
+        sorted = false;
+        break;
+      }
+  std::vector<int32_t> copy;
+  const int32_t* src = h_ci;
+  if (!sorted) {
+    copy.assign(h_ci, h_ci + nnz);
+    for (int32_t r = 0; r < nrows; ++r) std::sort(copy.begin() + h_rp[r], copy.begin() + h_rp[r + 1]);
+    src = copy.data();
+  }
+  GBO_HIP_CHECK(hipMalloc((void**)d_rp, sizeof(int32_t) * ((size_t)nrows + 1)));
+  GBO_HIP_CHECK(hipMalloc((void**)d_ci, sizeof(int32_t) * std::max<int64_t>(nnz, 1)));
+  GBO_HIP_CHECK(hipMemcpy(*d_rp, h_rp, sizeof(int32_t) * ((size_t)nrows + 1), hipMemcpyHostToDevice));
+  if (nnz) GBO_HIP_CHECK(hipMemcpy(*d_ci, src, sizeof(int32_t) * nnz, hipMemcpyHostToDevice));
+  return GBO_OK;
+}
+
+int gbo_csr2bsr_nnzb(int bs, const int32_t* h_rp, const int32_t* h_ci, int32_t nrows, int32_t ncols,
+                     int64_t* nblocks) {
+  GBO_TRY_BEGIN
+  if (!bs_ok(bs)) return set_error(GBO_EUNSUPPORTED, "block size %d not supported (8, 16, 32, 64)", bs);
+  if (!nblocks) return set_error(GBO_EINVAL, "null nblocks");
+  int32_t *d_rp = nullptr, *d_ci = nullptr, *d_brp = nullptr;
+  int rc = upload_csr(h_rp, h_ci, nrows, ncols, &d_rp, &d_ci);
+  const int32_t nbr = (nrows + bs - 1) / bs;
+  if (rc == GBO_OK && hipMalloc((void**)&d_brp, sizeof(int32_t) * ((size_t)nbr + 1)) != hipSuccess)
+    rc = set_error(GBO_ENOMEM, "allocation failed");
+  if (rc == GBO_OK) rc = gbo_csr2bsr_nnzb_dev(bs, d_rp, d_ci, nrows, ncols, d_brp, nblocks, nullptr);
+  (void)hipFree(d_rp);
+  (void)hipFree(d_ci);
+  (void)hipFree(d_brp);
+  return rc;
+  GBO_TRY_END
+}
+
+int gbo_csr2bsr_batch(int bs, int dir, const int32_t* h_rp, const int32_t* h_ci, int32_t nrows,
+                      int32_t ncols, int64_t nnz, int32_t* d_bsrRowPtr, int32_t* d_bsrColInd,
+                      void* d_tiles, int32_t nblockrows, int64_t nblocks, gbo_stream_t stream) {
+  GBO_TRY_BEGIN
+  if (!bs_ok(bs)) return set_error(GBO_EUNSUPPORTED, "block size %d not supported (8, 16, 32, 64)", bs);
+  if (!h_rp || h_rp[nrows] != nnz) return set_error(GBO_EINVAL, "gbo_csr2bsr_batch: nnz != rowptr[nrows]");
+  if (nblockrows != (nrows + bs - 1) / bs) return set_error(GBO_EINVAL, "gbo_csr2bsr_batch: nblockrows != ceil(nrows/bs)");
+  int32_t *d_rp = nullptr, *d_ci = nullptr;
+  int rc = upload_csr(h_rp, h_ci, nrows, ncols, &d_rp, &d_ci);
+  int64_t got = 0;
+  if (rc == GBO_OK) rc = gbo_csr2bsr_nnzb_dev(bs, d_rp, d_ci, nrows, ncols, d_bsrRowPtr, &got, stream);
+  if (rc == GBO_OK && got != nblocks)
+    rc = set_error(GBO_EINVAL, "gbo_csr2bsr_batch: nblocks %lld does not match the %lld tiles of the matrix",
+                   (long long)nblocks, (long long)got);
+  if (rc == GBO_OK) rc = gbo_csr2bsr_fill_dev(bs, dir, d_rp, d_ci, nrows, ncols, d_bsrRowPtr, d_bsrColInd, d_tiles, stream);
+  if (rc == GBO_OK) GBO_HIP_CHECK(hipStreamSynchronize(stream));
+  (void)hipFree(d_rp);
+  (void)hipFree(d_ci);
+  return rc;
+  GBO_TRY_END
+}
+
+int gbo_bsr_create(int bs, int dir, const int32_t* h_rp, const int32_t* h_ci, int32_t nrows, int32_t ncols,
+                   gbo_bsr* out, gbo_stream_t stream) {
+  GBO_TRY_BEGIN
+  int32_t *d_rp = nullptr, *d_ci = nullptr;
+  int rc = upload_csr(h_rp, h_ci, nrows, ncols, &d_rp, &d_ci);
+  if (rc == GBO_OK) rc = gbo_bsr_create_dev(bs, dir, d_rp, d_ci, nrows, ncols, out, stream);
+  if (rc == GBO_OK) GBO_HIP_CHECK(hipStreamSynchronize(stream));
+  (void)hipFree(d_rp);
+  (void)hipFree(d_ci);
+  return rc;
+  GBO_TRY_END
+}
+
+void gbo_bsr_plan_free(gbo_bsr* A) {
+  if (!A) return;
+  if (A->plan) (void)hipFree(A->plan);
+  A->plan = nullptr;
+  A->plan_chunk = 0;
+  A->plan_nchunks = 0;
+}
+
+void gbo_bsr_free(gbo_bsr* A) {
+  if (!A) return;
+  gbo_bsr_plan_free(A);
+  if (A->rowptr) (void)hipFree(A->rowptr);
+  if (A->colind) (void)hipFree(A->colind);
+  if (A->tiles) (void)hipFree(A->tiles);
+  A->rowptr = nullptr;
+  A->colind = nullptr;
+  A->tiles = nullptr;
+  A->nblocks = 0;
+}
+
+int gbo_bsr_plan(gbo_bsr* A, int32_t chunk_tiles, gbo_stream_t stream) {
+  int rc = validate_bsr(A, -1, "gbo_bsr_plan");
+  if (rc) return rc;
+  if (chunk_tiles < 0) return set_error(GBO_EINVAL, "gbo_bsr_plan: negative chunk");
+  const int32_t chunk = chunk_tiles > 0 ? chunk_tiles : default_chunk(A->bs);
+  const int64_t nch = std::max<int64_t>(1, (A->nblocks + chunk - 1) / chunk);
+  gbo_bsr_plan_free(A);
+  GBO_HIP_CHECK(hipMalloc((void**)&A->plan, sizeof(int32_t) * nch));
+  k_plan<<<(unsigned)((nch + 255) / 256), 256, 0, stream>>>(A->rowptr, A->nblockrows, chunk, (int32_t)nch, A->plan);
+  GBO_LAUNCH_CHECK();
+  A->plan_chunk = chunk;
+  A->plan_nchunks = (int32_t)nch;
+  return GBO_OK;
+}
+
+int gbo_csr2bsr_values_u32(const gbo_bsr* A, const int32_t* d_rp, const int32_t* d_ci, const uint32_t* d_val,
+                           uint32_t* d_tval, int64_t* d_woff, gbo_stream_t stream) {
+  int rc = validate_bsr(A, 0, "gbo_csr2bsr_values_u32");
+  if (rc) return rc;
+  if (!d_rp || !d_val || !d_tval || !d_woff) return set_error(GBO_EINVAL, "gbo_csr2bsr_values_u32: null argument");
+  hipStream_t st = stream;
+  const int64_t T = A->nblocks;
+  int64_t* cnt = nullptr;
+  GBO_HIP_CHECK(hipMallocAsync((void**)&cnt, sizeof(int64_t) * std::max<int64_t>(T, 1), st));
+  if (T > 0) {
+    const unsigned blocks = (unsigned)((T + 255) / 256);
+    switch (A->bs) {
+      case 8: k_tile_popc<8><<<blocks, 256, 0, st>>>((const uint8_t*)A->tiles, T, cnt); break;
+      case 16: k_tile_popc<16><<<blocks, 256, 0, st>>>((const uint16_t*)A->tiles, T, cnt); break;
+      case 32: k_tile_popc<32><<<blocks, 256, 0, st>>>((const uint32_t*)A->tiles, T, cnt); break;
+      case 64: k_tile_popc<64><<<blocks, 256, 0, st>>>((const uint64_t*)A->tiles, T, cnt); break;
+    }
+    GBO_LAUNCH_CHECK();
+  }
+  rc = scan_exclusive<int64_t, int64_t>(cnt, d_woff, T, st);
+  (void)hipFreeAsync(cnt, st);
+  if (rc) return rc;
+  return dispatch_pack<PACK_VALUES>(A->bs, d_rp, d_ci, A->nrows, A->ncols, A->nblockrows, A->rowptr, nullptr,
+                                    nullptr, d_val, d_woff, d_tval, nullptr, st);
+}
+
+int gbo_pack_vec(int bs, const float* d_x, void* d_xbits, int32_t n, gbo_stream_t stream) {
+  if (!bs_ok(bs)) return set_error(GBO_EUNSUPPORTED, "block size %d not supported (8, 16, 32, 64)", bs);
+  if (n < 0 || (n > 0 && (!d_x || !d_xbits))) return set_error(GBO_EINVAL, "gbo_pack_vec: bad argument");
+  const int32_t nw = (n + bs - 1) / bs;
+  if (nw == 0) return GBO_OK;
+  const unsigned blocks = (unsigned)((nw + 255) / 256);
+  switch (bs) {
+    case 8: k_pack_vec<8><<<blocks, 256, 0, stream>>>(d_x, (uint8_t*)d_xbits, n, nw); break;
+    case 16: k_pack_vec<16><<<blocks, 256, 0, stream>>>(d_x, (uint16_t*)d_xbits, n, nw); break;
+    case 32: k_pack_vec<32><<<blocks, 256, 0, stream>>>(d_x, (uint32_t*)d_xbits, n, nw); break;
+    case 64: k_pack_vec<64><<<blocks, 256, 0, stream>>>(d_x, (uint64_t*)d_xbits, n, nw); break;
+  }
+  GBO_LAUNCH_CHECK();
+  return GBO_OK;
+}
+
+}  // extern "C"

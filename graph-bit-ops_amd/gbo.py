@@ -1,0 +1,537 @@
+"""Python host mirror of the graph-bit-ops engine (ctypes over libgbo.so).
+
+The reference (carolc0708/graph-bit-ops) exposes its path at source level:
+``csr2bsr_batch_<N>(h_csrRowPtr, h_csrColInd, nrows, ncols, nnz, d_bsrRowPtr,
+d_bsrColInd, d_bsrVal, blocksize, nblockrows, nblocks)``
+(algorithm/csr2bsr_batch_bsrbmv.cu:180,406,621,786) followed by bmv/bmm kernel
+launches, all driven from C++ ``main`` functions.  This module binds the C ABI
+that replaces those entry points (include/gbo.h) and keeps the reference's
+names and argument meaning where one exists (``csr2bsr_batch``,
+``bsrbmv``/``bsrbmm`` operations, per-algorithm drivers).
+
+Only numpy and ctypes are needed; device memory is managed through the
+library's own HIP wrappers (``DeviceBuffer``).  Every call goes to the HIP
+kernels in libgbo.so; there is no CPU fallback -- a missing library raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+from ctypes import POINTER, byref, c_char_p, c_double, c_float, c_int, c_int32, c_int64, c_size_t, c_uint32, c_uint64, c_void_p
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgbo.so")
+_LIB = None
+
+GBO_OK = 0
+ERROR_NAMES = {
+    -1: "GBO_EINVAL", -2: "GBO_ENOMEM", -3: "GBO_EDEVICE", -4: "GBO_EIO",
+    -5: "GBO_EFORMAT", -6: "GBO_ERANGE", -7: "GBO_ENOCONV", -8: "GBO_EUNSUPPORTED",
+}
+BLOCK_SIZES = (8, 16, 32, 64)
+WORD_DTYPE = {8: np.uint8, 16: np.uint16, 32: np.uint32, 64: np.uint64}
+INF_U32 = 0xFFFFFFFF
+
+
+class GboError(RuntimeError):
+    """A negative GBO_E* status returned through the C ABI."""
+
+    def __init__(self, code: int, message: str):
+        super().__init__(f"{ERROR_NAMES.get(code, code)}: {message}")
+        self.code = code
+
+
+class BsrStruct(ctypes.Structure):
+    """Mirror of ``gbo_bsr`` in include/gbo.h."""
+
+    _fields_ = [
+        ("bs", c_int32), ("dir", c_int32), ("nrows", c_int32), ("ncols", c_int32),
+        ("nblockrows", c_int32), ("nblockcols", c_int32), ("nblocks", c_int64),
+        ("rowptr", c_void_p), ("colind", c_void_p), ("tiles", c_void_p),
+        ("plan", c_void_p), ("plan_chunk", c_int32), ("plan_nchunks", c_int32),
+    ]
+
+
+def header_symbols(header_path: str | None = None) -> list[str]:
+    """Names of every function declared in include/gbo.h."""
+    if header_path is None:
+        header_path = os.path.join(_HERE, "..", "include", "gbo.h")
+    text = open(header_path).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(gbo_[a-z0-9_]+)\s*\(", text)))
+
+
+def _declare(L):
+    P = c_void_p
+    i32p = POINTER(c_int32)
+    i64p = POINTER(c_int64)
+    sigs = {
+        "gbo_last_error": ([], c_char_p),
+        "gbo_version": ([], c_char_p),
+        "gbo_device_count": ([POINTER(c_int)], c_int),
+        "gbo_set_device": ([c_int], c_int),
+        "gbo_malloc": ([POINTER(c_void_p), c_size_t], c_int),
+        "gbo_free": ([P], c_int),
+        "gbo_memcpy_h2d": ([P, P, c_size_t], c_int),
+        "gbo_memcpy_d2h": ([P, P, c_size_t], c_int),
+        "gbo_memcpy_d2d": ([P, P, c_size_t], c_int),
+        "gbo_memset": ([P, c_int, c_size_t], c_int),
+        "gbo_synchronize": ([], c_int),
+        "gbo_stream_create": ([POINTER(c_void_p)], c_int),
+        "gbo_stream_destroy": ([P], c_int),
+        "gbo_stream_synchronize": ([P], c_int),
+        "gbo_event_create": ([POINTER(c_void_p)], c_int),
+        "gbo_event_destroy": ([P], c_int),
+        "gbo_event_record": ([P, P], c_int),
+        "gbo_event_elapsed": ([POINTER(c_float), P, P], c_int),
+        "gbo_free_host": ([P], None),
+        "gbo_mtx_read": ([c_char_p, c_int, c_int, i32p, i32p, i64p, POINTER(P), POINTER(P), POINTER(P)], c_int),
+        "gbo_mtx_write": ([c_char_p, c_int32, c_int32, P, P], c_int),
+        "gbo_gen_er": ([c_int32, c_double, c_uint64, i64p, POINTER(P), POINTER(P)], c_int),
+        "gbo_gen_er_m": ([c_int32, c_int64, c_uint64, i64p, POINTER(P), POINTER(P)], c_int),
+        "gbo_gen_rmat": ([c_int32, c_int32, c_double, c_double, c_double, c_uint64, i64p, POINTER(P), POINTER(P)], c_int),
+        "gbo_gen_king": ([c_int32, i64p, POINTER(P), POINTER(P)], c_int),
+        "gbo_gen_edge_weights": ([c_int32, P, P, c_uint32, c_uint32, c_uint64, P], c_int),
+        "gbo_csr_transpose": ([c_int32, c_int32, P, P, POINTER(P), POINTER(P)], c_int),
+        "gbo_csr_tril": ([c_int32, P, P, c_int, i64p, POINTER(P), POINTER(P)], c_int),
+        "gbo_csr2bsr_nnzb": ([c_int, P, P, c_int32, c_int32, i64p], c_int),
+        "gbo_csr2bsr_batch": ([c_int, c_int, P, P, c_int32, c_int32, c_int64, P, P, P, c_int32, c_int64, P], c_int),
+        "gbo_csr2bsr_nnzb_dev": ([c_int, P, P, c_int32, c_int32, P, i64p, P], c_int),
+        "gbo_csr2bsr_fill_dev": ([c_int, c_int, P, P, c_int32, c_int32, P, P, P, P], c_int),
+        "gbo_bsr_create_dev": ([c_int, c_int, P, P, c_int32, c_int32, POINTER(BsrStruct), P], c_int),
+        "gbo_bsr_create": ([c_int, c_int, P, P, c_int32, c_int32, POINTER(BsrStruct), P], c_int),
+        "gbo_bsr_free": ([POINTER(BsrStruct)], None),
+        "gbo_bsr_plan": ([POINTER(BsrStruct), c_int32, P], c_int),
+        "gbo_bsr_plan_free": ([POINTER(BsrStruct)], None),
+        "gbo_csr2bsr_values_u32": ([POINTER(BsrStruct), P, P, P, P, P, P], c_int),
+        "gbo_pack_vec": ([c_int, P, P, c_int32, P], c_int),
+        "gbo_bsrbmv_count": ([POINTER(BsrStruct), P, P, c_int, P], c_int),
+        "gbo_bsrbmv_bfs_step": ([POINTER(BsrStruct), P, P, P, P, P], c_int),
+        "gbo_bsrbmv_minplus": ([POINTER(BsrStruct), P, P, P, P, P], c_int),
+        "gbo_bsrbmv_sum_f32": ([POINTER(BsrStruct), P, P, P], c_int),
+        "gbo_bsrbmv_min_i32": ([POINTER(BsrStruct), P, P, P], c_int),
+        "gbo_bsrbmm_tc": ([POINTER(BsrStruct), POINTER(BsrStruct), P, P], c_int),
+        "gbo_bsrbmm_sum": ([POINTER(BsrStruct), POINTER(BsrStruct), P, P], c_int),
+        "gbo_bfs": ([POINTER(BsrStruct), c_int32, P, P, c_int32, i32p, P], c_int),
+        "gbo_sssp": ([POINTER(BsrStruct), P, P, c_int32, P, i32p, P], c_int),
+        "gbo_pagerank": ([POINTER(BsrStruct), P, c_float, c_double, c_int32, P, i32p, POINTER(c_double), P], c_int),
+        "gbo_cc": ([POINTER(BsrStruct), P, i32p, P], c_int),
+        "gbo_tc": ([c_int, c_int32, P, P, POINTER(c_uint64), P], c_int),
+    }
+    for name, (args, res) in sigs.items():
+        fn = getattr(L, name, None)
+        if fn is None:
+            continue
+        fn.argtypes = args
+        fn.restype = res
+    return L
+
+
+def lib():
+    """The loaded libgbo.so (raises if it has not been built)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                               "(make -C graph-bit-ops_amd/csrc)")
+        _LIB = _declare(ctypes.CDLL(LIB_PATH))
+    return _LIB
+
+
+def _check(rc: int):
+    if rc != GBO_OK:
+        raise GboError(rc, lib().gbo_last_error().decode(errors="replace"))
+
+
+def _ptr(a):
+    """Device pointer of a DeviceBuffer, host pointer of a numpy array, or None."""
+    if a is None:
+        return None
+    if isinstance(a, DeviceBuffer):
+        return a.ptr
+    if isinstance(a, np.ndarray):
+        assert a.flags["C_CONTIGUOUS"]
+        return a.ctypes.data
+    if isinstance(a, int):
+        return a
+    raise TypeError(type(a))
+
+
+def device_count() -> int:
+    n = c_int(0)
+    _check(lib().gbo_device_count(byref(n)))
+    return n.value
+
+
+def synchronize():
+    _check(lib().gbo_synchronize())
+
+
+class DeviceBuffer:
+    """An owned device allocation (hipMalloc through the library)."""
+
+    def __init__(self, nbytes: int):
+        p = c_void_p()
+        _check(lib().gbo_malloc(byref(p), max(int(nbytes), 1)))
+        self.ptr = p.value
+        self.nbytes = int(nbytes)
+
+    @classmethod
+    def from_numpy(cls, arr: np.ndarray) -> "DeviceBuffer":
+        arr = np.ascontiguousarray(arr)
+        buf = cls(arr.nbytes)
+        if arr.nbytes:
+            _check(lib().gbo_memcpy_h2d(buf.ptr, arr.ctypes.data, arr.nbytes))
+        return buf
+
+    @classmethod
+    def zeros(cls, nbytes: int) -> "DeviceBuffer":
+        buf = cls(nbytes)
+        _check(lib().gbo_memset(buf.ptr, 0, max(int(nbytes), 1)))
+        return buf
+
+    def to_numpy(self, dtype, count: int | None = None) -> np.ndarray:
+        dtype = np.dtype(dtype)
+        if count is None:
+            count = self.nbytes // dtype.itemsize
+        out = np.empty(count, dtype=dtype)
+        if out.nbytes:
+            _check(lib().gbo_memcpy_d2h(out.ctypes.data, self.ptr, out.nbytes))
+        return out
+
+    def copy_from(self, arr: np.ndarray):
+        arr = np.ascontiguousarray(arr)
+        assert arr.nbytes <= self.nbytes
+        if arr.nbytes:
+            _check(lib().gbo_memcpy_h2d(self.ptr, arr.ctypes.data, arr.nbytes))
+
+    def free(self):
+        if getattr(self, "ptr", None):
+            lib().gbo_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+# ----------------------------------------------------------------- CSR host
+class Csr:
+    """Host CSR (int32 rowptr/colind), as produced by the readers/generators."""
+
+    def __init__(self, nrows: int, ncols: int, rowptr: np.ndarray, colind: np.ndarray, values=None):
+        self.nrows = int(nrows)
+        self.ncols = int(ncols)
+        self.rowptr = np.ascontiguousarray(rowptr, dtype=np.int32)
+        self.colind = np.ascontiguousarray(colind, dtype=np.int32)
+        self.values = values
+
+    @property
+    def nnz(self) -> int:
+        return int(self.rowptr[-1])
+
+    def outdeg(self) -> np.ndarray:
+        return np.diff(self.rowptr).astype(np.int32)
+
+    def transpose(self) -> "Csr":
+        tp, ti = c_void_p(), c_void_p()
+        _check(lib().gbo_csr_transpose(self.nrows, self.ncols, _ptr(self.rowptr), _ptr(self.colind), byref(tp), byref(ti)))
+        return Csr(self.ncols, self.nrows, *_take(tp, self.ncols + 1, ti, self.nnz))
+
+    def tril(self, strict: bool = True) -> "Csr":
+        nnz = c_int64()
+        lp, li = c_void_p(), c_void_p()
+        _check(lib().gbo_csr_tril(self.nrows, _ptr(self.rowptr), _ptr(self.colind), int(strict), byref(nnz), byref(lp), byref(li)))
+        return Csr(self.nrows, self.ncols, *_take(lp, self.nrows + 1, li, nnz.value))
+
+
+def _take(p_rowptr, nrp, p_colind, nnz):
+    """Copy library-allocated host arrays into numpy and release them."""
+    L = lib()
+    rp = np.ctypeslib.as_array(ctypes.cast(p_rowptr, POINTER(c_int32)), shape=(nrp,)).copy()
+    ci = (np.ctypeslib.as_array(ctypes.cast(p_colind, POINTER(c_int32)), shape=(nnz,)).copy()
+          if nnz > 0 else np.zeros(0, np.int32))
+    L.gbo_free_host(p_rowptr)
+    L.gbo_free_host(p_colind)
+    return rp, ci
+
+
+def mtx_read(path: str, semantics: int = 1, directed: int = 0, values: bool = False) -> Csr:
+    """Read a Matrix Market file. semantics 0 = mmio_highlevel.h, 1 = readMtx.hpp."""
+    nr, nc, nnz = c_int32(), c_int32(), c_int64()
+    rp, ci, val = c_void_p(), c_void_p(), c_void_p()
+    _check(lib().gbo_mtx_read(os.fsencode(path), semantics, directed, byref(nr), byref(nc), byref(nnz),
+                              byref(rp), byref(ci), byref(val) if values else None))
+    vals = None
+    if values:
+        vals = (np.ctypeslib.as_array(ctypes.cast(val, POINTER(c_float)), shape=(nnz.value,)).copy()
+                if nnz.value > 0 else np.zeros(0, np.float32))
+        lib().gbo_free_host(val)
+    rpa, cia = _take(rp, nr.value + 1, ci, nnz.value)
+    return Csr(nr.value, nc.value, rpa, cia, vals)
+
+
+def mtx_write(path: str, csr: Csr):
+    _check(lib().gbo_mtx_write(os.fsencode(path), csr.nrows, csr.ncols, _ptr(csr.rowptr), _ptr(csr.colind)))
+
+
+def gen_er(n: int, p: float, seed: int) -> Csr:
+    nnz = c_int64()
+    rp, ci = c_void_p(), c_void_p()
+    _check(lib().gbo_gen_er(n, p, seed, byref(nnz), byref(rp), byref(ci)))
+    return Csr(n, n, *_take(rp, n + 1, ci, nnz.value))
+
+
+def gen_er_m(n: int, m: int, seed: int) -> Csr:
+    nnz = c_int64()
+    rp, ci = c_void_p(), c_void_p()
+    _check(lib().gbo_gen_er_m(n, m, seed, byref(nnz), byref(rp), byref(ci)))
+    return Csr(n, n, *_take(rp, n + 1, ci, nnz.value))
+
+
+def gen_rmat(scale: int, edgefactor: int = 16, a: float = 0.57, b: float = 0.19, c: float = 0.19, seed: int = 1) -> Csr:
+    nnz = c_int64()
+    rp, ci = c_void_p(), c_void_p()
+    _check(lib().gbo_gen_rmat(scale, edgefactor, a, b, c, seed, byref(nnz), byref(rp), byref(ci)))
+    n = 1 << scale
+    return Csr(n, n, *_take(rp, n + 1, ci, nnz.value))
+
+
+def gen_king(m: int) -> Csr:
+    nnz = c_int64()
+    rp, ci = c_void_p(), c_void_p()
+    _check(lib().gbo_gen_king(m, byref(nnz), byref(rp), byref(ci)))
+    return Csr(m * m, m * m, *_take(rp, m * m + 1, ci, nnz.value))
+
+
+def edge_weights(csr: Csr, lo: int = 1, hi: int = 64, seed: int = 1) -> np.ndarray:
+    w = np.empty(csr.nnz, np.uint32)
+    _check(lib().gbo_gen_edge_weights(csr.nrows, _ptr(csr.rowptr), _ptr(csr.colind), lo, hi, seed, _ptr(w)))
+    return w
+
+
+# ----------------------------------------------------------------- bit-BSR
+class BsrMatrix:
+    """A device-resident bit-BSR matrix (owns its gbo_bsr allocation)."""
+
+    def __init__(self, s: BsrStruct):
+        self.s = s
+
+    @classmethod
+    def from_csr(cls, csr: Csr, bs: int, direction: int = 0, plan: bool = True) -> "BsrMatrix":
+        s = BsrStruct()
+        _check(lib().gbo_bsr_create(bs, direction, _ptr(csr.rowptr), _ptr(csr.colind), csr.nrows, csr.ncols, byref(s), None))
+        m = cls(s)
+        if plan:
+            m.plan()
+        return m
+
+    @classmethod
+    def from_device_csr(cls, d_rowptr: DeviceBuffer, d_colind: DeviceBuffer, nrows: int, ncols: int, bs: int,
+                        direction: int = 0, plan: bool = True) -> "BsrMatrix":
+        s = BsrStruct()
+        _check(lib().gbo_bsr_create_dev(bs, direction, d_rowptr.ptr, d_colind.ptr, nrows, ncols, byref(s), None))
+        m = cls(s)
+        if plan:
+            m.plan()
+        return m
+
+    def plan(self, chunk_tiles: int = 0):
+        _check(lib().gbo_bsr_plan(byref(self.s), chunk_tiles, None))
+        return self
+
+    @property
+    def ref(self):
+        return byref(self.s)
+
+    bs = property(lambda self: self.s.bs)
+    nrows = property(lambda self: self.s.nrows)
+    ncols = property(lambda self: self.s.ncols)
+    nblockrows = property(lambda self: self.s.nblockrows)
+    nblockcols = property(lambda self: self.s.nblockcols)
+    nblocks = property(lambda self: self.s.nblocks)
+
+    def tile_bytes(self) -> int:
+        return self.s.nblocks * self.s.bs * self.s.bs // 8
+
+    def host_arrays(self):
+        """(rowptr, colind, tiles) copied to the host."""
+        L = lib()
+        s = self.s
+        rp = np.empty(s.nblockrows + 1, np.int32)
+        ci = np.empty(s.nblocks, np.int32)
+        tw = np.empty(s.nblocks * s.bs, WORD_DTYPE[s.bs])
+        _check(L.gbo_memcpy_d2h(rp.ctypes.data, s.rowptr, rp.nbytes))
+        if s.nblocks:
+            _check(L.gbo_memcpy_d2h(ci.ctypes.data, s.colind, ci.nbytes))
+            _check(L.gbo_memcpy_d2h(tw.ctypes.data, s.tiles, tw.nbytes))
+        return rp, ci, tw
+
+    def free(self):
+        if getattr(self, "s", None) is not None:
+            lib().gbo_bsr_free(byref(self.s))
+            self.s = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def csr2bsr_batch(csr: Csr, bs: int, direction: int = 0):
+    """Reference-shaped packing call: sizes with gbo_csr2bsr_nnzb (the
+    cusparseXcsr2bsrNnz step), allocates d_bsrRowPtr/d_bsrColInd/d_bsrVal like
+    the reference caller (bsrbmv/benchmark-bsrbmv.cu:1036-1048), then packs.
+    Returns (d_rowptr, d_colind, d_tiles, nblockrows, nblocks)."""
+    L = lib()
+    nb = c_int64()
+    _check(L.gbo_csr2bsr_nnzb(bs, _ptr(csr.rowptr), _ptr(csr.colind), csr.nrows, csr.ncols, byref(nb)))
+    nbr = (csr.nrows + bs - 1) // bs
+    d_rp = DeviceBuffer(4 * (nbr + 1))
+    d_ci = DeviceBuffer(4 * max(nb.value, 1))
+    d_t = DeviceBuffer(max(nb.value * bs * bs // 8, 8))
+    _check(L.gbo_csr2bsr_batch(bs, direction, _ptr(csr.rowptr), _ptr(csr.colind), csr.nrows, csr.ncols, csr.nnz,
+                               d_rp.ptr, d_ci.ptr, d_t.ptr, nbr, nb.value, None))
+    return d_rp, d_ci, d_t, nbr, nb.value
+
+
+def pack_vec(bs: int, x: np.ndarray) -> np.ndarray:
+    """ToBit{N}Row on the GPU: float vector -> bit words (element > 0)."""
+    x = np.ascontiguousarray(x, np.float32)
+    nw = (len(x) + bs - 1) // bs
+    dx = DeviceBuffer.from_numpy(x)
+    dw = DeviceBuffer(max(nw, 1) * bs // 8)
+    _check(lib().gbo_pack_vec(bs, dx.ptr, dw.ptr, len(x), None))
+    return dw.to_numpy(WORD_DTYPE[bs], nw)
+
+
+def _dev(a, dtype=None) -> DeviceBuffer:
+    if isinstance(a, DeviceBuffer):
+        return a
+    a = np.ascontiguousarray(a if dtype is None else np.asarray(a, dtype))
+    return DeviceBuffer.from_numpy(a)
+
+
+def bsrbmv_count(A: BsrMatrix, xbits, y_init: np.ndarray | None = None) -> np.ndarray:
+    """int32 y = A * x over bit words; with y_init the product is added (+=)."""
+    dx = _dev(xbits, WORD_DTYPE[A.bs])
+    if y_init is not None:
+        dy = DeviceBuffer.from_numpy(np.asarray(y_init, np.int32))
+    else:
+        dy = DeviceBuffer(4 * max(A.nrows, 1))
+    _check(lib().gbo_bsrbmv_count(A.ref, dx.ptr, dy.ptr, int(y_init is not None), None))
+    return dy.to_numpy(np.int32, A.nrows)
+
+
+def bsrbmv_bfs_step(A: BsrMatrix, frontier, visited):
+    """(next bit words, popcount) of one masked pull-BFS level."""
+    W = WORD_DTYPE[A.bs]
+    df, dv = _dev(frontier, W), _dev(visited, W)
+    dn = DeviceBuffer(max(A.nblockrows, 1) * A.bs // 8)
+    dc = DeviceBuffer(8)
+    _check(lib().gbo_bsrbmv_bfs_step(A.ref, df.ptr, dv.ptr, dn.ptr, dc.ptr, None))
+    return dn.to_numpy(W, A.nblockrows), int(dc.to_numpy(np.int64, 1)[0])
+
+
+class TileValues:
+    """Per-nonzero uint32 values in tile bit order (+ per-tile offsets)."""
+
+    def __init__(self, A: BsrMatrix, csr: Csr, values: np.ndarray):
+        d_rp = DeviceBuffer.from_numpy(csr.rowptr)
+        d_ci = DeviceBuffer.from_numpy(csr.colind)
+        d_v = DeviceBuffer.from_numpy(np.asarray(values, np.uint32))
+        self.tval = DeviceBuffer(4 * max(csr.nnz, 1))
+        self.woff = DeviceBuffer(8 * (A.nblocks + 1))
+        _check(lib().gbo_csr2bsr_values_u32(A.ref, d_rp.ptr, d_ci.ptr, d_v.ptr, self.tval.ptr, self.woff.ptr, None))
+        synchronize()
+        self.count = int(self.woff.to_numpy(np.int64)[A.nblocks]) if A.nblocks else 0
+
+
+def bsrbmv_minplus(A: BsrMatrix, dist, weights: TileValues | None = None) -> np.ndarray:
+    dd = _dev(dist, np.uint32)
+    do = DeviceBuffer(4 * max(A.nrows, 1))
+    w = weights.tval.ptr if weights else None
+    wo = weights.woff.ptr if weights else None
+    _check(lib().gbo_bsrbmv_minplus(A.ref, w, wo, dd.ptr, do.ptr, None))
+    return do.to_numpy(np.uint32, A.nrows)
+
+
+def bsrbmv_sum_f32(A: BsrMatrix, x) -> np.ndarray:
+    dx = _dev(x, np.float32)
+    dy = DeviceBuffer(4 * max(A.nrows, 1))
+    _check(lib().gbo_bsrbmv_sum_f32(A.ref, dx.ptr, dy.ptr, None))
+    return dy.to_numpy(np.float32, A.nrows)
+
+
+def bsrbmv_min_i32(A: BsrMatrix, x) -> np.ndarray:
+    dx = _dev(x, np.int32)
+    dy = DeviceBuffer(4 * max(A.nrows, 1))
+    _check(lib().gbo_bsrbmv_min_i32(A.ref, dx.ptr, dy.ptr, None))
+    return dy.to_numpy(np.int32, A.nrows)
+
+
+def bsrbmm_tc(L_row: BsrMatrix, LT_col: BsrMatrix) -> int:
+    d = DeviceBuffer(8)
+    _check(lib().gbo_bsrbmm_tc(L_row.ref, LT_col.ref, d.ptr, None))
+    return int(d.to_numpy(np.uint64, 1)[0])
+
+
+def bsrbmm_sum(A_row: BsrMatrix, B_col: BsrMatrix) -> int:
+    d = DeviceBuffer(8)
+    _check(lib().gbo_bsrbmm_sum(A_row.ref, B_col.ref, d.ptr, None))
+    synchronize()
+    return int(d.to_numpy(np.uint64, 1)[0])
+
+
+# ----------------------------------------------------------------- drivers
+def bfs(AT: BsrMatrix, source: int = 0, max_levels: int = 4096):
+    """(levels int32[n], per-level discovered counts)."""
+    dl = DeviceBuffer(4 * max(AT.nrows, 1))
+    sizes = np.zeros(max_levels, np.int64)
+    nl = c_int32()
+    _check(lib().gbo_bfs(AT.ref, source, dl.ptr, _ptr(sizes), max_levels, byref(nl), None))
+    return dl.to_numpy(np.int32, AT.nrows), sizes[: min(nl.value, max_levels)].copy()
+
+
+def sssp(AT: BsrMatrix, source: int = 0, weights: TileValues | None = None):
+    """(uint32 distances, iterations); INF = 0xFFFFFFFF."""
+    dd = DeviceBuffer(4 * max(AT.nrows, 1))
+    it = c_int32()
+    w = weights.tval.ptr if weights else None
+    wo = weights.woff.ptr if weights else None
+    _check(lib().gbo_sssp(AT.ref, w, wo, source, dd.ptr, byref(it), None))
+    return dd.to_numpy(np.uint32, AT.nrows), it.value
+
+
+def pagerank(AT: BsrMatrix, outdeg: np.ndarray, alpha: float = 0.85, tol: float = 1e-7, max_iter: int = 1000,
+             strict: bool = False):
+    """(float32 scores, iterations, last L1 change).  Reaching max_iter first
+    (GBO_ENOCONV) raises only when strict=True; the iterate is returned."""
+    do = DeviceBuffer.from_numpy(np.asarray(outdeg, np.int32))
+    dp = DeviceBuffer(4 * max(AT.nrows, 1))
+    it = c_int32()
+    delta = c_double()
+    rc = lib().gbo_pagerank(AT.ref, do.ptr, alpha, tol, max_iter, dp.ptr, byref(it), byref(delta), None)
+    if rc != GBO_OK and (strict or rc != -7):
+        _check(rc)
+    return dp.to_numpy(np.float32, AT.nrows), it.value, delta.value
+
+
+def cc(A: BsrMatrix):
+    """(int32 labels = smallest vertex id of the component, iterations)."""
+    dl = DeviceBuffer(4 * max(A.nrows, 1))
+    it = c_int32()
+    _check(lib().gbo_cc(A.ref, dl.ptr, byref(it), None))
+    return dl.to_numpy(np.int32, A.nrows), it.value
+
+
+def tc(csr: Csr, bs: int) -> int:
+    out = c_uint64()
+    _check(lib().gbo_tc(bs, csr.nrows, _ptr(csr.rowptr), _ptr(csr.colind), byref(out), None))
+    return out.value

@@ -1,0 +1,137 @@
+"""GPU driver parity: BFS levels, SSSP distances, PageRank, CC labels and
+triangle counts against textbook CPU algorithms and closed forms."""
+import math
+
+import numpy as np
+import pytest
+
+from graphs import cases, complete_graph
+
+pytestmark = pytest.mark.gpu
+BS = (8, 16, 32, 64)
+
+
+@pytest.fixture(scope="module")
+def graph_cases(gbo):
+    return cases(gbo)
+
+
+def sources(g):
+    deg = np.diff(g.rowptr)
+    s = [0, g.nrows - 1]
+    nz = np.nonzero(deg)[0]
+    if len(nz):
+        s.append(int(nz[len(nz) // 2]))
+    return sorted(set(s))
+
+
+@pytest.mark.parametrize("bs", BS)
+def test_bfs_levels(gbo, po, graph_cases, bs):
+    for name, g in graph_cases:
+        A = gbo.BsrMatrix.from_csr(g, bs)
+        rp, ci, tw = A.host_arrays()
+        for s in sources(g):
+            lv, sizes = gbo.bfs(A, s)
+            want = po.bfs(g.nrows, g.rowptr, g.colind, s)
+            assert np.array_equal(lv, want), (name, s)
+            # per-level discovered counts == histogram of levels (the
+            # reference's per-level "succ" line)
+            hist = np.bincount(want[want >= 0])
+            assert np.array_equal(sizes, hist), (name, s)
+            olv, osz = po.bfs_bsr(bs, g.nrows, rp, ci, tw, s)
+            assert np.array_equal(olv, want) and np.array_equal(osz, hist), (name, s)
+
+
+def test_bfs_closed_forms(gbo):
+    from graphs import path_graph, star_graph
+    rp, ci = path_graph(1000)
+    lv, _ = gbo.bfs(gbo.BsrMatrix.from_csr(gbo.Csr(1000, 1000, rp, ci), 64), 0)
+    assert np.array_equal(lv, np.arange(1000))
+    rp, ci = star_graph(513)
+    lv, sizes = gbo.bfs(gbo.BsrMatrix.from_csr(gbo.Csr(513, 513, rp, ci), 8), 0)
+    assert lv[0] == 0 and np.all(lv[1:] == 1) and list(sizes) == [1, 512]
+
+
+@pytest.mark.parametrize("bs", BS)
+def test_sssp(gbo, po, graph_cases, bs):
+    for name, g in graph_cases:
+        A = gbo.BsrMatrix.from_csr(g, bs)
+        s = sources(g)[-1]
+        d, _ = gbo.sssp(A, s)
+        lv = po.bfs(g.nrows, g.rowptr, g.colind, s)
+        want_unit = np.where(lv < 0, gbo.INF_U32, lv).astype(np.uint32)
+        assert np.array_equal(d, want_unit), name
+        w = gbo.edge_weights(g, 1, 64, seed=5)
+        tv = gbo.TileValues(A, g, w)
+        dw, _ = gbo.sssp(A, s, tv)
+        assert np.array_equal(dw, po.dijkstra(g.nrows, g.rowptr, g.colind, w, s)), name
+
+
+@pytest.mark.parametrize("bs", BS)
+def test_pagerank(gbo, po, graph_cases, bs):
+    for name, g in graph_cases:
+        if g.nnz == 0:
+            continue
+        A = gbo.BsrMatrix.from_csr(g, bs)
+        p, it, delta = gbo.pagerank(A, g.outdeg(), 0.85, 1e-8, 300)
+        want, _ = po.pagerank(g.nrows, g.rowptr, g.colind, g.outdeg(), 0.85, 1e-14, 100000)
+        # FP32 iterate vs FP64 fixpoint of the same formula: L1 <= 1e-6
+        assert np.abs(p.astype(np.float64) - want).sum() <= 1e-6, (name, it, delta)
+
+
+def test_pagerank_closed_forms(gbo):
+    from graphs import sym_csr
+    n = 640
+    u = np.arange(n)
+    rp, ci = sym_csr(n, u, (u + 1) % n)  # cycle: uniform
+    p, _, _ = gbo.pagerank(gbo.BsrMatrix.from_csr(gbo.Csr(n, n, rp, ci), 32), np.diff(rp), 0.85, 1e-10, 1000)
+    np.testing.assert_allclose(p, 1.0 / n, rtol=1e-5)
+    rp, ci = complete_graph(40)
+    p, _, _ = gbo.pagerank(gbo.BsrMatrix.from_csr(gbo.Csr(40, 40, rp, ci), 16), np.diff(rp), 0.85, 1e-10, 1000)
+    np.testing.assert_allclose(p, 1.0 / 40, rtol=1e-5)
+
+
+@pytest.mark.parametrize("bs", BS)
+def test_cc(gbo, po, graph_cases, bs):
+    for name, g in graph_cases:
+        A = gbo.BsrMatrix.from_csr(g, bs)
+        lab, _ = gbo.cc(A)
+        want = po.cc(g.nrows, g.rowptr, g.colind)
+        assert np.array_equal(lab, want), name
+
+
+@pytest.mark.parametrize("bs", BS)
+def test_tc_closed_forms(gbo, bs):
+    for m in (2, 3, 17, 70):
+        assert gbo.tc(gbo.gen_king(m), bs) == 4 * (m - 1) ** 2
+    for n in (3, 10, 65):
+        rp, ci = complete_graph(n)
+        assert gbo.tc(gbo.Csr(n, n, rp, ci), bs) == math.comb(n, 3)
+
+
+@pytest.mark.parametrize("bs", BS)
+def test_tc_vs_oracle(gbo, po, graph_cases, bs):
+    for name, g in graph_cases:
+        want = po.tc(g.nrows, g.rowptr, g.colind)
+        assert gbo.tc(g, bs) == want, name
+
+
+@pytest.mark.parametrize("bs", BS)
+def test_bsrbmm_operands(gbo, po, bs):
+    """gbo_bsrbmm_tc / gbo_bsrbmm_sum on explicit (L row, L^T col) operands,
+    as the reference's tc/bsrbmm drivers build them, against the oracle's
+    restatement of the v5 traversal and the closed form sum_c colcount_c(L)^2."""
+    for g in (gbo.gen_rmat(10, 16, seed=2), gbo.gen_er(700, 0.03, 4)):
+        L = g.tril(strict=True)
+        LT = L.transpose()
+        Lr = gbo.BsrMatrix.from_csr(L, bs, 0)
+        Lc = gbo.BsrMatrix.from_csr(LT, bs, 1)
+        arp, aci, atw = Lr.host_arrays()
+        brp, bci, btw = Lc.host_arrays()
+        want = po.bsrbmm_masked(bs, arp, aci, atw, brp, bci, btw)
+        assert want == po.tc(g.nrows, g.rowptr, g.colind)
+        assert gbo.bsrbmm_tc(Lr, Lc) == want
+        colcount = np.bincount(L.colind, minlength=g.nrows).astype(np.int64)
+        s = int((colcount ** 2).sum())
+        assert po.bsrbmm_sum(bs, arp, aci, atw, brp, bci, btw) == s
+        assert gbo.bsrbmm_sum(Lr, Lc) == s

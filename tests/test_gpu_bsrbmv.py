@@ -1,0 +1,107 @@
+"""GPU bsrbmv parity: count / BFS step / min-plus / sum / min through the C ABI
+against the oracle (bit-exact for integer ops, FP32 sum within a stated
+tolerance of the FP64 oracle and bitwise reproducible).  Each op runs with
+the default plan and with tiny chunks so that most block rows are split
+across wavefronts (atomic and ordered-fix-up combine paths)."""
+import numpy as np
+import pytest
+
+from conftest import bernoulli_x
+from graphs import cases
+
+pytestmark = pytest.mark.gpu
+BS = (8, 16, 32, 64)
+CHUNKS = (0, 3)  # 0 = default chunk; 3 tiles = split rows everywhere
+
+
+@pytest.fixture(scope="module")
+def graph_cases(gbo):
+    return cases(gbo)
+
+
+def packed(gbo, g, bs, chunk):
+    A = gbo.BsrMatrix.from_csr(g, bs, 0, plan=False)
+    A.plan(chunk)
+    return A, A.host_arrays()
+
+
+@pytest.mark.parametrize("bs", BS)
+@pytest.mark.parametrize("chunk", CHUNKS)
+def test_count(gbo, po, graph_cases, bs, chunk):
+    for name, g in graph_cases:
+        A, (rp, ci, tw) = packed(gbo, g, bs, chunk)
+        x = bernoulli_x(g.ncols, 0.1, 3)
+        xb = gbo.pack_vec(bs, x)
+        y = gbo.bsrbmv_count(A, xb)
+        assert np.array_equal(y, po.count(bs, g.nrows, rp, ci, tw, xb)), name
+        # accumulate (+=) semantics of the reference kernels
+        y0 = np.arange(g.nrows, dtype=np.int32)
+        assert np.array_equal(gbo.bsrbmv_count(A, xb, y_init=y0), y + y0), name
+
+
+def test_count_dense_bruteforce(gbo):
+    g = gbo.gen_er(1024, 0.02, 0)
+    D = np.zeros((1024, 1024), np.int64)
+    for r in range(1024):
+        D[r, g.colind[g.rowptr[r]:g.rowptr[r + 1]]] = 1
+    x = bernoulli_x(1024, 0.1, 0)
+    want = D @ x.astype(np.int64)
+    for bs in BS:
+        A = gbo.BsrMatrix.from_csr(g, bs)
+        assert np.array_equal(gbo.bsrbmv_count(A, gbo.pack_vec(bs, x)), want)
+
+
+@pytest.mark.parametrize("bs", BS)
+@pytest.mark.parametrize("chunk", CHUNKS)
+def test_bfs_step(gbo, po, graph_cases, bs, chunk):
+    rng = np.random.default_rng(bs + chunk)
+    for name, g in graph_cases:
+        A, (rp, ci, tw) = packed(gbo, g, bs, chunk)
+        for pf, pv in ((0.01, 0.0), (0.2, 0.5), (0.5, 0.95)):
+            f = gbo.pack_vec(bs, (rng.random(g.ncols) < pf).astype(np.float32))
+            v = gbo.pack_vec(bs, (rng.random(g.nrows) < pv).astype(np.float32))
+            nx, c = gbo.bsrbmv_bfs_step(A, f, v)
+            onx, oc = po.bfs_step(bs, g.nrows, rp, ci, tw, f, v)
+            assert np.array_equal(nx, onx), name
+            assert c == oc, name
+
+
+@pytest.mark.parametrize("bs", BS)
+@pytest.mark.parametrize("chunk", CHUNKS)
+def test_minplus(gbo, po, graph_cases, bs, chunk):
+    rng = np.random.default_rng(100 + bs)
+    for name, g in graph_cases:
+        A, (rp, ci, tw) = packed(gbo, g, bs, chunk)
+        dist = rng.integers(0, 1000, g.nrows).astype(np.uint32)
+        dist[rng.random(g.nrows) < 0.3] = gbo.INF_U32
+        dist[rng.random(g.nrows) < 0.01] = gbo.INF_U32 - 3  # saturation
+        assert np.array_equal(gbo.bsrbmv_minplus(A, dist), po.minplus(bs, g.nrows, rp, ci, tw, dist)), name
+        w = gbo.edge_weights(g, 1, 64, seed=9)
+        tv = gbo.TileValues(A, g, w)
+        otv, owoff = po.pack_values(bs, g.nrows, g.rowptr, g.colind, w, rp, ci, tw)
+        got = gbo.bsrbmv_minplus(A, dist, tv)
+        assert np.array_equal(got, po.minplus(bs, g.nrows, rp, ci, tw, dist, otv, owoff)), name
+
+
+@pytest.mark.parametrize("bs", BS)
+@pytest.mark.parametrize("chunk", CHUNKS)
+def test_sum_f32(gbo, po, graph_cases, bs, chunk):
+    rng = np.random.default_rng(200 + bs)
+    for name, g in graph_cases:
+        A, (rp, ci, tw) = packed(gbo, g, bs, chunk)
+        x = rng.random(g.ncols).astype(np.float32)
+        y = gbo.bsrbmv_sum_f32(A, x)
+        want = po.sum_f64(bs, g.nrows, rp, ci, tw, x.astype(np.float64))
+        # FP32 accumulation of <= max-degree terms in (0,1): relative 1e-5
+        np.testing.assert_allclose(y, want, rtol=1e-5, atol=1e-6, err_msg=name)
+        assert np.array_equal(y.view(np.uint32), gbo.bsrbmv_sum_f32(A, x).view(np.uint32)), name
+
+
+@pytest.mark.parametrize("bs", BS)
+@pytest.mark.parametrize("chunk", CHUNKS)
+def test_min_i32(gbo, po, graph_cases, bs, chunk):
+    rng = np.random.default_rng(300 + bs)
+    for name, g in graph_cases:
+        A, (rp, ci, tw) = packed(gbo, g, bs, chunk)
+        x = rng.integers(-2**31, 2**31 - 1, g.ncols).astype(np.int32)
+        assert np.array_equal(gbo.bsrbmv_min_i32(A, x), po.min_i32(bs, g.nrows, rp, ci, tw, x)), name
