@@ -74,6 +74,23 @@ def time_count(gbo, A, dx, dy, steps, warmup, timer):
     return timer.stop_ms() / steps
 
 
+def copy_ceiling(gbo, timer, nbytes=1 << 30, reps=10):
+    """HBM copy ceiling (read + write bytes / time) of a plain copy kernel."""
+    L = gbo.lib()
+    a = gbo.DeviceBuffer(nbytes)
+    b = gbo.DeviceBuffer(nbytes)
+    gbo._check(L.gbo_memset(a.ptr, 1, nbytes))
+    gbo._check(L.gbo_stream_copy(b.ptr, a.ptr, nbytes, None))
+    gbo.synchronize()
+    timer.start()
+    for _ in range(reps):
+        gbo._check(L.gbo_stream_copy(b.ptr, a.ptr, nbytes, None))
+    ms = timer.stop_ms() / reps
+    a.free()
+    b.free()
+    return round(2 * nbytes / (ms * 1e-3) / 1e9, 1)
+
+
 def cpu_baseline(gbo, A, xbits, n, target_s):
     """Oracle OpenMP sweep over a contiguous sample of block rows."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -128,6 +145,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-algos", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--profile", action="store_true",
+                    help="short run for rocprofv3: only the count sweeps and the copy kernel")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -145,6 +164,10 @@ def main():
     d_ci = gbo.DeviceBuffer.from_numpy(g.colind)
     outdeg = g.outdeg()
     timer = Timer(gbo)
+    copy_gbs = copy_ceiling(gbo, timer)
+    log(f"[bench] copy-kernel ceiling {copy_gbs} GB/s")
+    if args.profile:
+        args.no_algos = args.no_cpu = True
     x = (np.random.default_rng(0).random(n) < 0.5).astype(np.float32)
     dxf = gbo.DeviceBuffer.from_numpy(x)
     dy = gbo.DeviceBuffer(4 * n)
@@ -209,7 +232,7 @@ def main():
                                f"symmetrized, dedup, seed 1), x=Bernoulli(0.5)",
                    "nnz": nnz, "bs": bs, "parallelism": "none (1 GPU)"},
         "roofline": {"bound": "hbm", "achieved": best["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": best["frac_hbm"], "traffic": None},
+                     "frac": best["frac_hbm"], "traffic": None, "copy_ceiling_GBs": copy_gbs},
         "cpu_baseline": cpu,
         "per_block_size": per_bs,
     }

@@ -18,6 +18,15 @@
 
 namespace gbo {
 
+// Neumaier's compensated addition: s + c carries the sum of all added terms
+// with the rounding error of each addition accumulated in c.
+__device__ __forceinline__ void kahan_add(float& s, float& c, float v) {
+  const float t = s + v;
+  if (fabsf(s) >= fabsf(v)) c += (s - t) + v;
+  else c += (v - t) + s;
+  s = t;
+}
+
 // Identity written for a block row with no tiles (lane-per-row, see k_spmv).
 template <int BS, int OP>
 __device__ __forceinline__ void write_empty_row(const SpmvArgs& a, int32_t R) {
@@ -172,24 +181,31 @@ __global__ __launch_bounds__(256) void k_spmv(const SpmvArgs a) {
           }
         }
       } else if constexpr (OP == OP_SUM) {
+        // Compensated (Neumaier) FP32 summation: the error of a row sum stays
+        // within a few ulps of the result whatever the row length (a hub
+        // row adds >10^5 terms).
         const float* __restrict__ x = (const float*)a.xval;
-        float acc = 0.0f;
+        float acc = 0.0f, comp = 0.0f;
         for (int64_t t = s + g; t < te; t += G) {
           const int32_t col = colind[t];
           uint64_t wd = (uint64_t)tiles[t * BS + r];
           while (wd) {
             const int p = __builtin_ctzll(wd);
-            acc += x[(int64_t)col * BS + (BS - 1 - p)];
+            kahan_add(acc, comp, x[(int64_t)col * BS + (BS - 1 - p)]);
             wd &= wd - 1;
           }
         }
-        for (int off = BS; off < kWave; off <<= 1) acc += __shfl_xor(acc, off);
+        for (int off = BS; off < kWave; off <<= 1) {
+          const float os = __shfl_xor(acc, off), oc = __shfl_xor(comp, off);
+          kahan_add(acc, comp, os);
+          comp += oc;
+        }
         if (g == 0 && i < a.nrows) {
           if (full) {
-            ((float*)a.out)[i] = a.scale * acc + a.bias;
+            ((float*)a.out)[i] = a.scale * (acc + comp) + a.bias;
           } else {
             const int slot = (rs < cb) ? 0 : 1;
-            a.partial[((int64_t)c * 2 + slot) * BS + r] = acc;
+            a.partial[((int64_t)c * 2 + slot) * BS + r] = acc + comp;
           }
         }
       } else if constexpr (OP == OP_MIN) {
@@ -259,10 +275,10 @@ __global__ __launch_bounds__(256) void k_sum_fixup(const SpmvArgs a) {
   const int32_t R = a.plan[c + 1];
   const int64_t rs = a.rowptr[R], re = a.rowptr[R + 1];
   if (!(rs < nb) || rs < c * a.chunk) return;  // not split here, or not the owner
-  float s = a.partial[(c * 2 + 1) * BS + r];
-  for (int64_t c2 = c + 1; c2 < a.nchunks && c2 * a.chunk < re; ++c2) s += a.partial[(c2 * 2 + 0) * BS + r];
+  float s = a.partial[(c * 2 + 1) * BS + r], comp = 0.0f;
+  for (int64_t c2 = c + 1; c2 < a.nchunks && c2 * a.chunk < re; ++c2) kahan_add(s, comp, a.partial[(c2 * 2 + 0) * BS + r]);
   const int64_t i = (int64_t)R * BS + r;
-  if (i < a.nrows) ((float*)a.out)[i] = a.scale * s + a.bias;
+  if (i < a.nrows) ((float*)a.out)[i] = a.scale * (s + comp) + a.bias;
 }
 
 template <int BS, int OP>

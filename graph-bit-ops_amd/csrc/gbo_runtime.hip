@@ -1,8 +1,32 @@
 // Thin C-ABI wrappers over the HIP runtime used by hosts without their own
 // GPU bindings (ctypes tests, the CLIs, bench.py).
+#include <algorithm>
+
 #include "gbo_device.hpp"
 
+namespace {
+
+// Grid-stride 16-byte copy: the plain copy used to measure the HBM copy
+// ceiling and to calibrate the FETCH_SIZE / WRITE_SIZE counters.
+__global__ __launch_bounds__(256) void k_stream_copy(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n16) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
+}  // namespace
+
 extern "C" {
+
+int gbo_stream_copy(void* d_dst, const void* d_src, size_t bytes, gbo_stream_t stream) {
+  if (bytes % 16 || ((uintptr_t)d_dst | (uintptr_t)d_src) % 16)
+    return gbo::set_error(GBO_EINVAL, "gbo_stream_copy: size and pointers must be 16-byte aligned");
+  const size_t n16 = bytes / 16;
+  if (n16 == 0) return GBO_OK;
+  const size_t blocks = std::min<size_t>((n16 + 255) / 256, 256 * 64);
+  k_stream_copy<<<(unsigned)blocks, 256, 0, stream>>>((const uint4*)d_src, (uint4*)d_dst, n16);
+  GBO_LAUNCH_CHECK();
+  return GBO_OK;
+}
 
 int gbo_device_count(int* count) {
   if (!count) return gbo::set_error(GBO_EINVAL, "gbo_device_count: null");

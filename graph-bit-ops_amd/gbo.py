@@ -88,6 +88,7 @@ def _declare(L):
         "gbo_event_record": ([P, P], c_int),
         "gbo_event_elapsed": ([POINTER(c_float), P, P], c_int),
         "gbo_free_host": ([P], None),
+        "gbo_stream_copy": ([P, P, c_size_t, P], c_int),
         "gbo_mtx_read": ([c_char_p, c_int, c_int, i32p, i32p, i64p, POINTER(P), POINTER(P), POINTER(P)], c_int),
         "gbo_mtx_write": ([c_char_p, c_int32, c_int32, P, P], c_int),
         "gbo_gen_er": ([c_int32, c_double, c_uint64, i64p, POINTER(P), POINTER(P)], c_int),

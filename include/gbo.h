@@ -95,6 +95,11 @@ int gbo_event_destroy(void* ev);
 int gbo_event_record(void* ev, gbo_stream_t stream);
 int gbo_event_elapsed(float* ms, void* start, void* stop);
 void gbo_free_host(void* h_ptr);
+/* Plain device-to-device copy kernel (16-byte vector loads/stores; size and
+ * pointers 16-byte aligned): measures the HBM copy ceiling reported next to
+ * the roofline and calibrates the FETCH_SIZE/WRITE_SIZE counters.  No
+ * reference counterpart. */
+int gbo_stream_copy(void* d_dst, const void* d_src, size_t bytes, gbo_stream_t stream);
 
 /* ------------------------------------------------- Matrix Market readers --
  * semantics 0: mmio_info + mmio_data (algorithm/mmio_highlevel.h:7,146):
