@@ -31,6 +31,10 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "bsrbmv GTEPS + HBM roofline frac, R-MAT s22; batched/adjoint vs plain on 1 GPU"
 
 
+def gbo_default_chunk(bs):
+    return {8: 4096, 16: 2048, 32: 512, 64: 256}[bs]
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -145,6 +149,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-algos", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--variants", default="0,1,2", help="bs 8/16 kernel variants to time (best is kept)")
     ap.add_argument("--profile", action="store_true",
                     help="short run for rocprofv3: only the count sweeps and the copy kernel")
     args = ap.parse_args()
@@ -181,12 +186,22 @@ def main():
         pack_s = time.time() - t
         dx = gbo.DeviceBuffer((A.nblockcols * bs) // 8)
         gbo._check(gbo.lib().gbo_pack_vec(bs, dxf.ptr, dx.ptr, n, None))
-        ms = time_count(gbo, A, dx, dy, args.steps, args.warmup, timer)
+        # autotune: chunk size and (bs 8/16) kernel variant, best time kept
+        tried = {}
+        variants = [int(v) for v in args.variants.split(",")] if bs <= 16 else [0]
+        for variant in variants:
+            for chunk in ((0,) if args.profile else (0, 2 * gbo_default_chunk(bs), gbo_default_chunk(bs) // 2)):
+                A.plan(chunk, variant)
+                tried[(variant, A.s.plan_chunk)] = time_count(gbo, A, dx, dy, args.steps, args.warmup, timer)
+        (bv, bc), ms = min(tried.items(), key=lambda kv: kv[1])
+        A.plan(bc, bv)
+        log(f"[bench] bs={bs} tuning (variant, chunk) -> ms: " + ", ".join(f"{k}: {v:.3f}" for k, v in tried.items()))
         byt = sweep_bytes(A, n)
         rec = {
             "bs": bs, "tiles": A.nblocks, "nnz_per_tile": round(nnz / max(A.nblocks, 1), 3),
             "tile_GB": round(A.tile_bytes() / 1e9, 3), "pack_s": round(pack_s, 3),
             "ms_per_sweep": round(ms, 4), "GTEPS": round(nnz / (ms * 1e-3) / 1e9, 3),
+            "variant": bv, "chunk": bc,
             "bytes_per_sweep": byt, "achieved_GBs": round(byt / (ms * 1e-3) / 1e9, 1),
             "frac_hbm": round(byt / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         }

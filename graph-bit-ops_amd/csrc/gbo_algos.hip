@@ -90,6 +90,7 @@ static int bfs_t(const gbo_bsr* A, int32_t source, int32_t* d_level, int64_t* si
   a.plan = pv.plan;
   a.chunk = pv.chunk;
   a.nchunks = pv.nchunks;
+  a.variant = pv.variant;
   a.count = d_cnt;
   int32_t lvl = 0;
   if (sizes && max_levels > 0) sizes[0] = 1;
@@ -320,6 +321,7 @@ int gbo_sssp(const gbo_bsr* AT, const uint32_t* d_w, const int64_t* d_woff, int3
   a.plan = pv.plan;
   a.chunk = pv.chunk;
   a.nchunks = pv.nchunks;
+  a.variant = pv.variant;
   a.w = d_w;
   a.woff = d_woff;
   int32_t it = 0;
@@ -383,6 +385,7 @@ int gbo_pagerank(const gbo_bsr* AT, const int32_t* d_outdeg, float alpha, double
   a.plan = pv.plan;
   a.chunk = pv.chunk;
   a.nchunks = pv.nchunks;
+  a.variant = pv.variant;
   a.partial = partial;
   a.scale = 1.0f;
   a.bias = (1.0f - alpha) / (float)n;
@@ -451,6 +454,7 @@ int gbo_cc(const gbo_bsr* A, int32_t* d_label, int32_t* h_iters, gbo_stream_t st
   a.plan = pv.plan;
   a.chunk = pv.chunk;
   a.nchunks = pv.nchunks;
+  a.variant = pv.variant;
   int32_t it = 0;
   while (true) {
     a.xval = gf;

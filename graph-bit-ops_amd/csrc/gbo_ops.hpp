@@ -16,6 +16,7 @@ struct PlanView {
   const int32_t* plan = nullptr;
   int32_t chunk = 0;
   int32_t nchunks = 0;
+  int32_t variant = 0;
   bool owned = false;
 };
 
@@ -40,6 +41,7 @@ struct SpmvArgs {
   int64_t T;
   const int32_t* plan;
   int32_t chunk, nchunks;
+  int32_t variant;      // bs 8/16 COUNT/BFS kernel variant
   const void* xbits;    // COUNT: x bits; BFS: frontier bits
   const void* visited;  // BFS: visited bits (per block row)
   void* out;            // COUNT int32 y / BFS next bits / MINPLUS uint32 / SUM float / MIN int32

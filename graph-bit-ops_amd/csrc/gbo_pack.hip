@@ -298,6 +298,7 @@ int acquire_plan(const gbo_bsr* A, PlanView* pv, hipStream_t st) {
     pv->plan = A->plan;
     pv->chunk = A->plan_chunk;
     pv->nchunks = A->plan_nchunks;
+    pv->variant = A->plan_variant;
     pv->owned = false;
     return GBO_OK;
   }
@@ -311,6 +312,7 @@ int acquire_plan(const gbo_bsr* A, PlanView* pv, hipStream_t st) {
   pv->plan = plan;
   pv->chunk = chunk;
   pv->nchunks = (int32_t)nch;
+  pv->variant = A->plan_variant;
   pv->owned = true;
   return GBO_OK;
 }
@@ -539,9 +541,14 @@ void gbo_bsr_free(gbo_bsr* A) {
 }
 
 int gbo_bsr_plan(gbo_bsr* A, int32_t chunk_tiles, gbo_stream_t stream) {
+  return gbo_bsr_plan_ex(A, chunk_tiles, A ? A->plan_variant : 0, stream);
+}
+
+int gbo_bsr_plan_ex(gbo_bsr* A, int32_t chunk_tiles, int32_t variant, gbo_stream_t stream) {
   int rc = validate_bsr(A, -1, "gbo_bsr_plan");
   if (rc) return rc;
   if (chunk_tiles < 0) return set_error(GBO_EINVAL, "gbo_bsr_plan: negative chunk");
+  if (variant < 0 || variant > 2) return set_error(GBO_EINVAL, "gbo_bsr_plan: variant must be 0, 1 or 2");
   const int32_t chunk = chunk_tiles > 0 ? chunk_tiles : default_chunk(A->bs);
   const int64_t nch = std::max<int64_t>(1, (A->nblocks + chunk - 1) / chunk);
   gbo_bsr_plan_free(A);
@@ -550,6 +557,7 @@ int gbo_bsr_plan(gbo_bsr* A, int32_t chunk_tiles, gbo_stream_t stream) {
   GBO_LAUNCH_CHECK();
   A->plan_chunk = chunk;
   A->plan_nchunks = (int32_t)nch;
+  A->plan_variant = variant;
   return GBO_OK;
 }
 

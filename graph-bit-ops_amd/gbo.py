@@ -52,6 +52,7 @@ class BsrStruct(ctypes.Structure):
         ("nblockrows", c_int32), ("nblockcols", c_int32), ("nblocks", c_int64),
         ("rowptr", c_void_p), ("colind", c_void_p), ("tiles", c_void_p),
         ("plan", c_void_p), ("plan_chunk", c_int32), ("plan_nchunks", c_int32),
+        ("plan_variant", c_int32),
     ]
 
 
@@ -107,6 +108,7 @@ def _declare(L):
         "gbo_bsr_free": ([POINTER(BsrStruct)], None),
         "gbo_bsr_plan": ([POINTER(BsrStruct), c_int32, P], c_int),
         "gbo_bsr_plan_free": ([POINTER(BsrStruct)], None),
+        "gbo_bsr_plan_ex": ([POINTER(BsrStruct), c_int32, c_int32, P], c_int),
         "gbo_csr2bsr_values_u32": ([POINTER(BsrStruct), P, P, P, P, P, P], c_int),
         "gbo_pack_vec": ([c_int, P, P, c_int32, P], c_int),
         "gbo_bsrbmv_count": ([POINTER(BsrStruct), P, P, c_int, P], c_int),
@@ -342,8 +344,9 @@ class BsrMatrix:
             m.plan()
         return m
 
-    def plan(self, chunk_tiles: int = 0):
-        _check(lib().gbo_bsr_plan(byref(self.s), chunk_tiles, None))
+    def plan(self, chunk_tiles: int = 0, variant: int | None = None):
+        v = self.s.plan_variant if variant is None else variant
+        _check(lib().gbo_bsr_plan_ex(byref(self.s), chunk_tiles, v, None))
         return self
 
     @property

@@ -69,6 +69,7 @@ typedef struct gbo_bsr {
   int32_t* plan;        /* device, plan_nchunks entries (first block row of each chunk) */
   int32_t plan_chunk;   /* tiles per chunk */
   int32_t plan_nchunks;
+  int32_t plan_variant; /* kernel variant for bs 8/16 (0 = default; see gbo_bsr_plan_ex) */
 } gbo_bsr;
 
 const char* gbo_last_error(void);
@@ -174,6 +175,12 @@ void gbo_bsr_free(gbo_bsr* A);
  * default for the block size) and record each chunk's first block row.
  * Allocates A->plan (released by gbo_bsr_free / gbo_bsr_plan_free). */
 int gbo_bsr_plan(gbo_bsr* A, int32_t chunk_tiles, gbo_stream_t stream);
+/* As gbo_bsr_plan, also selecting the bs 8/16 kernel variant of the COUNT
+ * and BFS products: 0 = one lane group per tile row word (default),
+ * 1 = one lane per tile, block rows one after another, 2 = one lane per
+ * tile, segmented over block rows (COUNT only; BFS uses 1).  Results are
+ * identical; only speed differs (the autotuner picks). */
+int gbo_bsr_plan_ex(gbo_bsr* A, int32_t chunk_tiles, int32_t variant, gbo_stream_t stream);
 void gbo_bsr_plan_free(gbo_bsr* A);
 /* Per-nonzero values in tile bit order (row-major, MSB-first inside a row
  * word) plus per-tile offsets woff[nblocks+1].  Duplicate CSR entries keep

@@ -19,17 +19,20 @@ def graph_cases(gbo):
     return cases(gbo)
 
 
-def packed(gbo, g, bs, chunk):
+def packed(gbo, g, bs, chunk, variant=0):
     A = gbo.BsrMatrix.from_csr(g, bs, 0, plan=False)
-    A.plan(chunk)
+    A.plan(chunk, variant)
     return A, A.host_arrays()
 
 
 @pytest.mark.parametrize("bs", BS)
 @pytest.mark.parametrize("chunk", CHUNKS)
-def test_count(gbo, po, graph_cases, bs, chunk):
+@pytest.mark.parametrize("variant", (0, 1, 2))
+def test_count(gbo, po, graph_cases, bs, chunk, variant):
+    if variant and bs > 16:
+        pytest.skip("kernel variants exist for bs 8/16 only")
     for name, g in graph_cases:
-        A, (rp, ci, tw) = packed(gbo, g, bs, chunk)
+        A, (rp, ci, tw) = packed(gbo, g, bs, chunk, variant)
         x = bernoulli_x(g.ncols, 0.1, 3)
         xb = gbo.pack_vec(bs, x)
         y = gbo.bsrbmv_count(A, xb)
@@ -53,10 +56,13 @@ def test_count_dense_bruteforce(gbo):
 
 @pytest.mark.parametrize("bs", BS)
 @pytest.mark.parametrize("chunk", CHUNKS)
-def test_bfs_step(gbo, po, graph_cases, bs, chunk):
+@pytest.mark.parametrize("variant", (0, 1))
+def test_bfs_step(gbo, po, graph_cases, bs, chunk, variant):
+    if variant and bs > 16:
+        pytest.skip("kernel variants exist for bs 8/16 only")
     rng = np.random.default_rng(bs + chunk)
     for name, g in graph_cases:
-        A, (rp, ci, tw) = packed(gbo, g, bs, chunk)
+        A, (rp, ci, tw) = packed(gbo, g, bs, chunk, variant)
         for pf, pv in ((0.01, 0.0), (0.2, 0.5), (0.5, 0.95)):
             f = gbo.pack_vec(bs, (rng.random(g.ncols) < pf).astype(np.float32))
             v = gbo.pack_vec(bs, (rng.random(g.nrows) < pv).astype(np.float32))
