@@ -123,6 +123,7 @@ def _declare(L):
         "gbo_pagerank": ([POINTER(BsrStruct), P, c_float, c_double, c_int32, P, i32p, POINTER(c_double), P], c_int),
         "gbo_cc": ([POINTER(BsrStruct), P, i32p, P], c_int),
         "gbo_tc": ([c_int, c_int32, P, P, POINTER(c_uint64), P], c_int),
+        "gbo_bfs_multisource": ([POINTER(BsrStruct), P, c_int32, P, i32p, P], c_int),
     }
     for name, (args, res) in sigs.items():
         fn = getattr(L, name, None)
@@ -539,3 +540,18 @@ def tc(csr: Csr, bs: int) -> int:
     out = c_uint64()
     _check(lib().gbo_tc(bs, csr.nrows, _ptr(csr.rowptr), _ptr(csr.colind), byref(out), None))
     return out.value
+
+
+# ----------------------------------------------------------------- batching
+def bfs_multisource(AT: BsrMatrix, sources, out: DeviceBuffer | None = None):
+    """Levels of up to 256 BFS runs in one batched pass: (int32 [nsrc][n], nlevels).
+    With `out` given (a device buffer of nsrc*n int32) the levels stay on the
+    device and only nlevels is returned."""
+    src = np.ascontiguousarray(sources, np.int32)
+    n = AT.nrows
+    d = out if out is not None else DeviceBuffer(4 * max(len(src) * n, 1))
+    nl = c_int32()
+    _check(lib().gbo_bfs_multisource(AT.ref, _ptr(src), len(src), d.ptr, byref(nl), None))
+    if out is not None:
+        return nl.value
+    return d.to_numpy(np.int32, len(src) * n).reshape(len(src), n), nl.value

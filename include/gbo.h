@@ -280,6 +280,16 @@ int gbo_cc(const gbo_bsr* A, int32_t* d_label, int32_t* h_iters, gbo_stream_t st
 int gbo_tc(int bs, int32_t nrows, const int32_t* h_rowptr, const int32_t* h_colind,
            uint64_t* h_count, gbo_stream_t stream);
 
+/* ------------------------------------------------------------ batching --
+ * Multi-source BFS: up to 256 BFS runs from h_sources[0..nsrc) in one pass
+ * per level, as a bit-BSR product with nsrc-bit frontier words per vertex
+ * (no reference counterpart; it batches the BFS of algorithm/bfs.cu:508-564).
+ * d_level is [nsrc][n] int32: d_level[s*n + v] = hop distance of v from
+ * source s, -1 if unreachable -- identical to nsrc separate gbo_bfs runs.
+ * *h_nlevels = number of levels including level 0. */
+int gbo_bfs_multisource(const gbo_bsr* AT, const int32_t* h_sources, int32_t nsrc, int32_t* d_level,
+                        int32_t* h_nlevels, gbo_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -135,3 +135,20 @@ def test_bsrbmm_operands(gbo, po, bs):
         s = int((colcount ** 2).sum())
         assert po.bsrbmm_sum(bs, arp, aci, atw, brp, bci, btw) == s
         assert gbo.bsrbmm_sum(Lr, Lc) == s
+
+
+@pytest.mark.parametrize("bs", BS)
+def test_bfs_multisource(gbo, po, graph_cases, bs):
+    """Batched BFS == per-source BFS (oracle queue BFS), bit-exact, for 1, 64,
+    100 and 256 sources including duplicates and isolated vertices."""
+    rng = np.random.default_rng(bs)
+    for name, g in graph_cases:
+        A = gbo.BsrMatrix.from_csr(g, bs)
+        for k in (1, 64, 100, 256):
+            src = rng.integers(0, g.nrows, k).astype(np.int32)
+            src[0] = src[-1]  # a duplicate
+            lv, nl = gbo.bfs_multisource(A, src)
+            for s in range(0, k, max(1, k // 9)):
+                want = po.bfs(g.nrows, g.rowptr, g.colind, int(src[s]))
+                assert np.array_equal(lv[s], want), (name, k, s)
+            assert nl == int(lv.max()) + 1, (name, k)
