@@ -1,0 +1,429 @@
+// Personalized PageRank over K vectors with a hand-written adjoint.
+// No reference counterpart: it generalises the PageRank of algorithm/pr.cu
+// (p' = alpha * P^T p + (1-alpha)/n, P = D^-1 A) to K personalization
+// vectors s_k and adds the reverse-mode solve.
+//
+//   forward:  p_k = alpha * P^T p_k + (1 - alpha) * s_k      (pull over A^T)
+//   adjoint:  lambda_k = alpha * P lambda_k + w_k            (pull over A)
+//   dL/ds_k = (1 - alpha) * lambda_k,
+//   dL/dalpha = sum_k lambda_k . (P^T p_k - s_k)
+// for L = sum_k w_k . p_k.  The adjoint needs only the converged p (no
+// checkpointing of the forward iterates).
+//
+// The product is a bit-BSR SpMM with a dense K-column right-hand side in
+// vertex-major layout X[n][K]: lane k of a wave owns column k, so every set
+// bit (r, c) of a tile costs one coalesced 4K-byte row read X[j][0..K).
+// Tile row words are fetched by lanes r < bs, the rows with bits are found
+// by a ballot, and each row's partial sum lives in register acc[r]
+// (uniform index).  Waves take tile-balanced chunks; rows cut by a chunk
+// boundary leave compensated partials that a fix-up kernel adds in chunk
+// order, so results are bitwise reproducible.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "gbo_device.hpp"
+#include "gbo_ops.hpp"
+
+namespace gbo {
+
+struct PprArgs {
+  const int32_t* rowptr;
+  const int32_t* colind;
+  const void* tiles;
+  int32_t nrows, nbr;
+  int64_t T;
+  const int32_t* plan;
+  int32_t chunk, nchunks;
+  int32_t K;                 // columns (<= 64 per launch; ld is the row stride)
+  int32_t ld;                // row stride of X/Y/Z (total columns)
+  const float* X;            // [n][ld] right-hand side (column block offset applied)
+  const float* colscale;     // per source vertex factor on X (nullable -> 1)
+  const float* rowscale;     // per output row factor (nullable -> 1)
+  float a;                   // Y = a * rowscale * (A X) + b * Z
+  float b;
+  const float* Z;            // nullable
+  float* Y;
+  float* partial;            // [nchunks][2][bs][64] split-row partials (sum + compensation)
+};
+
+__device__ __forceinline__ void nadd(float& s, float& c, float v) {
+  const float t = s + v;
+  if (fabsf(s) >= fabsf(v)) c += (s - t) + v;
+  else c += (v - t) + s;
+  s = t;
+}
+
+template <int BS>
+__device__ __forceinline__ void ppr_store_row(const PprArgs& A, int64_t R, int lane, const float* acc, const float* cmp,
+                                              bool full, int slot, int c) {
+  for (int r = 0; r < BS; ++r) {
+    const int64_t i = R * BS + r;
+    if (i >= A.nrows) break;
+    const float v = acc[r] + cmp[r];
+    if (full) {
+      if (lane < A.K) {
+        const float rs = A.rowscale ? A.rowscale[i] : 1.0f;
+        float y = A.a * rs * v;
+        if (A.Z) y += A.b * A.Z[i * A.ld + lane];
+        A.Y[i * A.ld + lane] = y;
+      }
+    } else {
+      A.partial[(((int64_t)c * 2 + slot) * BS + r) * kWave + lane] = v;
+    }
+  }
+}
+
+template <int BS>
+__global__ __launch_bounds__(256) void k_ppr_spmm(const PprArgs A) {
+  using W = word_t<BS>;
+  const int lane = lane_id();
+  const int c = (int)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave);
+  if (c >= A.nchunks) return;
+  const int64_t cb = (int64_t)c * A.chunk;
+  const int64_t ce = min(cb + (int64_t)A.chunk, A.T);
+  const bool last = (c == A.nchunks - 1);
+  const W* __restrict__ tiles = (const W*)A.tiles;
+  const bool col_ok = lane < A.K;
+
+  for (int32_t R0 = A.plan[c]; R0 < A.nbr;) {
+    const int32_t Rl = R0 + lane;
+    int32_t rsl = 0, rel = 0;
+    if (Rl < A.nbr) {
+      rsl = A.rowptr[Rl];
+      rel = A.rowptr[Rl + 1];
+    }
+    const bool belong = (Rl < A.nbr) && (last || (int64_t)rsl < ce);
+    const uint64_t bmask = __ballot(belong);
+    uint64_t todo = bmask;  // empty rows are written with the epilogue too
+    while (todo) {
+      const int j = __builtin_ctzll(todo);
+      todo &= todo - 1;
+      const int64_t R = R0 + j;
+      const int64_t rs = __shfl(rsl, j), re = __shfl(rel, j);
+      const int64_t s = rs > cb ? rs : cb;
+      const int64_t te = re < ce ? re : ce;
+      const bool full = (rs >= cb) && (re <= ce);
+      float acc[BS], cmp[BS];
+#pragma unroll
+      for (int r = 0; r < BS; ++r) acc[r] = cmp[r] = 0.0f;
+      for (int64_t t = s; t < te; ++t) {
+        const int32_t col = A.colind[t];
+        const uint64_t word = (lane < BS) ? (uint64_t)tiles[t * BS + lane] : 0ull;
+        uint64_t rows = __ballot(word != 0);
+        while (rows) {
+          const int r = __builtin_ctzll(rows);
+          rows &= rows - 1;
+          uint64_t wr = (uint64_t)__shfl((long long)word, r);
+          while (wr) {
+            const int p = __builtin_ctzll(wr);
+            wr &= wr - 1;
+            const int64_t v = (int64_t)col * BS + (BS - 1 - p);
+            float x = col_ok ? A.X[v * A.ld + lane] : 0.0f;
+            if (A.colscale) x *= A.colscale[v];
+            nadd(acc[r], cmp[r], x);  // r is wave-uniform
+          }
+        }
+      }
+      ppr_store_row<BS>(A, R, lane, acc, cmp, full, (rs < cb) ? 0 : 1, c);
+    }
+    if (__popcll(bmask) < kWave) break;
+    R0 += kWave;
+  }
+}
+
+template <int BS>
+__global__ __launch_bounds__(256) void k_ppr_fixup(const PprArgs A) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = (int)(idx % kWave);
+  const int64_t cr = idx / kWave;  // (chunk, row) pairs
+  const int64_t c = cr / BS;
+  const int r = (int)(cr % BS);
+  if (c + 1 >= A.nchunks) return;
+  const int64_t nb = (c + 1) * A.chunk;
+  const int32_t R = A.plan[c + 1];
+  const int64_t rs = A.rowptr[R], re = A.rowptr[R + 1];
+  if (!(rs < nb) || rs < c * A.chunk) return;
+  float s = A.partial[((c * 2 + 1) * BS + r) * kWave + lane], cmp = 0.0f;
+  for (int64_t c2 = c + 1; c2 < A.nchunks && c2 * A.chunk < re; ++c2)
+    nadd(s, cmp, A.partial[((c2 * 2 + 0) * BS + r) * kWave + lane]);
+  const int64_t i = (int64_t)R * BS + r;
+  if (i < A.nrows && lane < A.K) {
+    const float rsc = A.rowscale ? A.rowscale[i] : 1.0f;
+    float y = A.a * rsc * (s + cmp);
+    if (A.Z) y += A.b * A.Z[i * A.ld + lane];
+    A.Y[i * A.ld + lane] = y;
+  }
+}
+
+// Deterministic column-wise reduction over a [n][ld] column block of
+// `cols` <= 64 columns; mode 0: sum |Y - P|, mode 1: sum L * (Y - P).  Each
+// block writes 64 partials; k_colred_final adds them in block order.
+__global__ __launch_bounds__(256) void k_colred(const float* __restrict__ Y, const float* __restrict__ P,
+                                                const float* __restrict__ L, int32_t n, int32_t ld, int32_t cols,
+                                                int mode, double* __restrict__ part) {
+  const int col = threadIdx.x & 63, rsub = threadIdx.x >> 6;
+  double acc = 0.0;
+  if (col < cols) {
+    for (int64_t i = (int64_t)blockIdx.x * 4 + rsub; i < n; i += (int64_t)gridDim.x * 4) {
+      const int64_t o = i * ld + col;
+      if (mode == 0) acc += fabs((double)Y[o] - (double)P[o]);
+      else acc += (double)L[o] * ((double)Y[o] - (double)P[o]);
+    }
+  }
+  __shared__ double sh[256];
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  if (threadIdx.x < 64)
+    part[(int64_t)blockIdx.x * 64 + threadIdx.x] =
+        sh[threadIdx.x] + sh[threadIdx.x + 64] + sh[threadIdx.x + 128] + sh[threadIdx.x + 192];
+}
+
+__global__ void k_colred_final(const double* __restrict__ part, int nb, double* out) {
+  const int k = threadIdx.x;
+  if (k >= 64) return;
+  double s = 0.0;
+  for (int b = 0; b < nb; ++b) s += part[(int64_t)b * 64 + k];
+  out[k] = s;
+}
+
+__global__ void k_invdeg(const int32_t* __restrict__ deg, float* __restrict__ inv, int32_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) inv[i] = deg[i] > 0 ? 1.0f / (float)deg[i] : 0.0f;
+}
+
+__global__ void k_scale_copy(const float* __restrict__ src, float* __restrict__ dst, int64_t n, float a) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = a * src[i];
+}
+
+// Plans with a chunk sized for the SpMM (one tile costs up to bs*K loads).
+struct PprPlan {
+  int32_t* plan = nullptr;
+  int32_t chunk = 0, nchunks = 0;
+  float* partial = nullptr;
+};
+
+__global__ void k_plan_ppr(const int32_t* __restrict__ rowptr, int32_t nbr, int32_t chunk, int32_t nchunks,
+                           int32_t* __restrict__ plan) {
+  const int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nchunks) return;
+  const int64_t b = (int64_t)c * chunk;
+  int32_t lo = 0, hi = nbr;
+  while (lo < hi) {
+    const int32_t mid = lo + (hi - lo) / 2;
+    if ((int64_t)rowptr[mid] < b) lo = mid + 1;
+    else hi = mid;
+  }
+  int32_t R = lo;
+  if ((int64_t)rowptr[R] > b) R -= 1;
+  if (R >= nbr) R = nbr > 0 ? nbr - 1 : 0;
+  if (R < 0) R = 0;
+  plan[c] = R;
+}
+
+static int make_ppr_plan(const gbo_bsr* M, PprPlan* P, hipStream_t st) {
+  P->chunk = 1024;
+  P->nchunks = (int32_t)std::max<int64_t>(1, (M->nblocks + P->chunk - 1) / P->chunk);
+  GBO_HIP_CHECK(hipMallocAsync((void**)&P->plan, sizeof(int32_t) * P->nchunks, st));
+  GBO_HIP_CHECK(hipMallocAsync((void**)&P->partial, sizeof(float) * 2 * (size_t)P->nchunks * M->bs * kWave, st));
+  k_plan_ppr<<<(P->nchunks + 255) / 256, 256, 0, st>>>(M->rowptr, M->nblockrows, P->chunk, P->nchunks, P->plan);
+  GBO_LAUNCH_CHECK();
+  return GBO_OK;
+}
+
+static void free_ppr_plan(PprPlan* P, hipStream_t st) {
+  if (P->plan) (void)hipFreeAsync(P->plan, st);
+  if (P->partial) (void)hipFreeAsync(P->partial, st);
+  P->plan = nullptr;
+  P->partial = nullptr;
+}
+
+// Y = a * rowscale * (M (colscale .* X)) + b * Z over all K columns.
+static int spmm(const gbo_bsr* M, const PprPlan& P, int32_t K, const float* X, const float* colscale,
+                const float* rowscale, float a, float b, const float* Z, float* Y, hipStream_t st) {
+  for (int32_t k0 = 0; k0 < K; k0 += kWave) {
+    PprArgs A{};
+    A.rowptr = M->rowptr;
+    A.colind = M->colind;
+    A.tiles = M->tiles;
+    A.nrows = M->nrows;
+    A.nbr = M->nblockrows;
+    A.T = M->nblocks;
+    A.plan = P.plan;
+    A.chunk = P.chunk;
+    A.nchunks = P.nchunks;
+    A.K = std::min(kWave, K - k0);
+    A.ld = K;
+    A.X = X + k0;
+    A.colscale = colscale;
+    A.rowscale = rowscale;
+    A.a = a;
+    A.b = b;
+    A.Z = Z ? Z + k0 : nullptr;
+    A.Y = Y + k0;
+    A.partial = P.partial;
+    const unsigned blocks = (unsigned)(((int64_t)P.nchunks + 3) / 4);
+    switch (M->bs) {
+      case 8: k_ppr_spmm<8><<<blocks, 256, 0, st>>>(A); break;
+      case 16: k_ppr_spmm<16><<<blocks, 256, 0, st>>>(A); break;
+      case 32: k_ppr_spmm<32><<<blocks, 256, 0, st>>>(A); break;
+      default: return set_error(GBO_EUNSUPPORTED, "PPR supports block sizes 8, 16, 32");
+    }
+    GBO_LAUNCH_CHECK();
+    if (P.nchunks > 1) {
+      const int64_t thr = (int64_t)(P.nchunks - 1) * M->bs * kWave;
+      const unsigned fb = (unsigned)((thr + 255) / 256);
+      switch (M->bs) {
+        case 8: k_ppr_fixup<8><<<fb, 256, 0, st>>>(A); break;
+        case 16: k_ppr_fixup<16><<<fb, 256, 0, st>>>(A); break;
+        case 32: k_ppr_fixup<32><<<fb, 256, 0, st>>>(A); break;
+      }
+      GBO_LAUNCH_CHECK();
+    }
+  }
+  return GBO_OK;
+}
+
+// Column-wise reduction over [n][K] in column blocks of 64 (deterministic).
+static int colreduce(const float* Y, const float* P, const float* L, int32_t n, int32_t K, int mode, double* d_part,
+                     double* d_out, double* h_out, hipStream_t st) {
+  constexpr int nb = 256;
+  for (int32_t k0 = 0; k0 < K; k0 += 64) {
+    const int32_t cols = std::min(64, K - k0);
+    k_colred<<<nb, 256, 0, st>>>(Y + k0, P + k0, L ? L + k0 : nullptr, n, K, cols, mode, d_part);
+    GBO_LAUNCH_CHECK();
+    k_colred_final<<<1, 64, 0, st>>>(d_part, nb, d_out);
+    GBO_LAUNCH_CHECK();
+    GBO_HIP_CHECK(hipMemcpyAsync(h_out + k0, d_out, sizeof(double) * cols, hipMemcpyDeviceToHost, st));
+  }
+  GBO_HIP_CHECK(hipStreamSynchronize(st));
+  return GBO_OK;
+}
+
+}  // namespace gbo
+
+using namespace gbo;
+
+extern "C" {
+
+int gbo_ppr_forward(const gbo_bsr* AT, const int32_t* d_outdeg, int32_t K, float alpha, const float* d_s, float* d_p,
+                    double tol, int32_t max_iter, int32_t* h_iters, double* h_delta, gbo_stream_t stream) {
+  int rc = validate_bsr(AT, 0, "gbo_ppr_forward");
+  if (rc) return rc;
+  if (AT->nrows != AT->ncols) return set_error(GBO_EINVAL, "gbo_ppr_forward: matrix must be square");
+  if (K < 1 || K > 64) return set_error(GBO_EINVAL, "gbo_ppr_forward: 1 <= K <= 64");
+  if (!(alpha > 0.0f && alpha < 1.0f) || max_iter < 1) return set_error(GBO_EINVAL, "gbo_ppr_forward: bad alpha/max_iter");
+  if (!d_outdeg || !d_s || !d_p) return set_error(GBO_EINVAL, "gbo_ppr_forward: null argument");
+  if (AT->bs > 32) return set_error(GBO_EUNSUPPORTED, "gbo_ppr_forward: block sizes 8, 16, 32");
+  hipStream_t st = stream;
+  const int32_t n = AT->nrows;
+  const int64_t nk = (int64_t)n * K;
+  float *inv = nullptr, *q = nullptr;
+  double* dpart = nullptr;
+  GBO_HIP_CHECK(hipMallocAsync((void**)&inv, sizeof(float) * std::max(n, 1), st));
+  GBO_HIP_CHECK(hipMallocAsync((void**)&q, sizeof(float) * std::max<int64_t>(nk, 1), st));
+  GBO_HIP_CHECK(hipMallocAsync((void**)&dpart, sizeof(double) * (256 * 64 + 64), st));
+  k_invdeg<<<(n + 255) / 256, 256, 0, st>>>(d_outdeg, inv, n);
+  GBO_LAUNCH_CHECK();
+  PprPlan P;
+  rc = make_ppr_plan(AT, &P, st);
+  // p_0 = s
+  if (rc == GBO_OK) GBO_HIP_CHECK(hipMemcpyAsync(d_p, d_s, sizeof(float) * nk, hipMemcpyDeviceToDevice, st));
+  std::vector<double> delta(K);
+  int32_t it = 0;
+  double dmax = 0.0;
+  float* cur = d_p;
+  float* nxt = q;
+  while (rc == GBO_OK && it < max_iter) {
+    rc = spmm(AT, P, K, cur, inv, nullptr, alpha, 1.0f - alpha, d_s, nxt, st);
+    if (rc) break;
+    rc = colreduce(nxt, cur, nullptr, n, K, 0, dpart, dpart + 256 * 64, delta.data(), st);
+    if (rc) break;
+    ++it;
+    std::swap(cur, nxt);
+    dmax = *std::max_element(delta.begin(), delta.end());
+    if (dmax < tol) break;
+  }
+  if (rc == GBO_OK && cur != d_p) GBO_HIP_CHECK(hipMemcpyAsync(d_p, cur, sizeof(float) * nk, hipMemcpyDeviceToDevice, st));
+  free_ppr_plan(&P, st);
+  (void)hipFreeAsync(inv, st);
+  (void)hipFreeAsync(q, st);
+  (void)hipFreeAsync(dpart, st);
+  GBO_HIP_CHECK(hipStreamSynchronize(st));
+  if (h_iters) *h_iters = it;
+  if (h_delta) *h_delta = dmax;
+  if (rc == GBO_OK && !(dmax < tol))
+    return set_error(GBO_ENOCONV, "gbo_ppr_forward: not converged after %d iterations (%.3e)", it, dmax);
+  return rc;
+}
+
+int gbo_ppr_adjoint(const gbo_bsr* A, const gbo_bsr* AT, const int32_t* d_outdeg, int32_t K, float alpha,
+                    const float* d_s, const float* d_p, const float* d_w, float* d_grad_s, double* h_grad_alpha,
+                    double tol, int32_t max_iter, int32_t* h_iters, gbo_stream_t stream) {
+  int rc = validate_bsr(A, 0, "gbo_ppr_adjoint(A)");
+  if (rc) return rc;
+  rc = validate_bsr(AT, 0, "gbo_ppr_adjoint(AT)");
+  if (rc) return rc;
+  if (A->nrows != A->ncols || AT->nrows != A->nrows) return set_error(GBO_EINVAL, "gbo_ppr_adjoint: shapes");
+  if (K < 1 || K > 64) return set_error(GBO_EINVAL, "gbo_ppr_adjoint: 1 <= K <= 64");
+  if (!(alpha > 0.0f && alpha < 1.0f) || max_iter < 1) return set_error(GBO_EINVAL, "gbo_ppr_adjoint: bad alpha/max_iter");
+  if (!d_outdeg || !d_s || !d_p || !d_w || !d_grad_s) return set_error(GBO_EINVAL, "gbo_ppr_adjoint: null argument");
+  if (A->bs > 32 || AT->bs > 32) return set_error(GBO_EUNSUPPORTED, "gbo_ppr_adjoint: block sizes 8, 16, 32");
+  hipStream_t st = stream;
+  const int32_t n = A->nrows;
+  const int64_t nk = (int64_t)n * K;
+  float *inv = nullptr, *lam = nullptr, *q = nullptr;
+  double* dpart = nullptr;
+  GBO_HIP_CHECK(hipMallocAsync((void**)&inv, sizeof(float) * std::max(n, 1), st));
+  GBO_HIP_CHECK(hipMallocAsync((void**)&lam, sizeof(float) * std::max<int64_t>(nk, 1) * 2, st));
+  q = lam + nk;
+  GBO_HIP_CHECK(hipMallocAsync((void**)&dpart, sizeof(double) * (256 * 64 + 64), st));
+  k_invdeg<<<(n + 255) / 256, 256, 0, st>>>(d_outdeg, inv, n);
+  GBO_LAUNCH_CHECK();
+  PprPlan PA, PT;
+  rc = make_ppr_plan(A, &PA, st);
+  if (rc == GBO_OK) rc = make_ppr_plan(AT, &PT, st);
+  if (rc == GBO_OK) GBO_HIP_CHECK(hipMemcpyAsync(lam, d_w, sizeof(float) * nk, hipMemcpyDeviceToDevice, st));
+  std::vector<double> delta(K), ga(K);
+  int32_t it = 0;
+  float* cur = lam;
+  float* nxt = q;
+  double dmax = 0.0;
+  while (rc == GBO_OK && it < max_iter) {
+    // lambda' = alpha * D^-1 A lambda + w
+    rc = spmm(A, PA, K, cur, nullptr, inv, alpha, 1.0f, d_w, nxt, st);
+    if (rc) break;
+    rc = colreduce(nxt, cur, nullptr, n, K, 0, dpart, dpart + 256 * 64, delta.data(), st);
+    if (rc) break;
+    ++it;
+    std::swap(cur, nxt);
+    dmax = *std::max_element(delta.begin(), delta.end());
+    if (dmax < tol) break;
+  }
+  if (rc == GBO_OK) {
+    // grad_s = (1 - alpha) * lambda
+    k_scale_copy<<<(unsigned)((nk + 255) / 256), 256, 0, st>>>(cur, d_grad_s, nk, 1.0f - alpha);
+    GBO_LAUNCH_CHECK();
+    // dL/dalpha = sum_k lambda_k . (P^T p_k - s_k)
+    rc = spmm(AT, PT, K, d_p, inv, nullptr, 1.0f, 0.0f, nullptr, nxt, st);
+    if (rc == GBO_OK) rc = colreduce(nxt, d_s, cur, n, K, 1, dpart, dpart + 256 * 64, ga.data(), st);
+    if (rc == GBO_OK && h_grad_alpha) {
+      double s = 0.0;
+      for (double v : ga) s += v;
+      *h_grad_alpha = s;
+    }
+  }
+  free_ppr_plan(&PA, st);
+  free_ppr_plan(&PT, st);
+  (void)hipFreeAsync(inv, st);
+  (void)hipFreeAsync(lam, st);
+  (void)hipFreeAsync(dpart, st);
+  GBO_HIP_CHECK(hipStreamSynchronize(st));
+  if (h_iters) *h_iters = it;
+  if (rc == GBO_OK && !(dmax < tol))
+    return set_error(GBO_ENOCONV, "gbo_ppr_adjoint: not converged after %d iterations (%.3e)", it, dmax);
+  return rc;
+}
+
+}  // extern "C"

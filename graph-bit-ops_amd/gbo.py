@@ -124,6 +124,10 @@ def _declare(L):
         "gbo_cc": ([POINTER(BsrStruct), P, i32p, P], c_int),
         "gbo_tc": ([c_int, c_int32, P, P, POINTER(c_uint64), P], c_int),
         "gbo_bfs_multisource": ([POINTER(BsrStruct), P, c_int32, P, i32p, P], c_int),
+        "gbo_ppr_forward": ([POINTER(BsrStruct), P, c_int32, c_float, P, P, c_double, c_int32, i32p,
+                             POINTER(c_double), P], c_int),
+        "gbo_ppr_adjoint": ([POINTER(BsrStruct), POINTER(BsrStruct), P, c_int32, c_float, P, P, P, P,
+                             POINTER(c_double), c_double, c_int32, i32p, P], c_int),
     }
     for name, (args, res) in sigs.items():
         fn = getattr(L, name, None)
@@ -555,3 +559,39 @@ def bfs_multisource(AT: BsrMatrix, sources, out: DeviceBuffer | None = None):
     if out is not None:
         return nl.value
     return d.to_numpy(np.int32, len(src) * n).reshape(len(src), n), nl.value
+
+
+# ------------------------------------------------------ personalized PageRank
+def ppr_forward(AT: BsrMatrix, outdeg, S, alpha: float = 0.85, tol: float = 1e-7, max_iter: int = 1000,
+                strict: bool = False):
+    """P[n][K] = personalized PageRank of the K columns of S[n][K] (float32).
+    Returns (P, iterations, last max-column L1 change)."""
+    S = np.ascontiguousarray(S, np.float32)
+    n, K = S.shape
+    do = DeviceBuffer.from_numpy(np.asarray(outdeg, np.int32))
+    ds = DeviceBuffer.from_numpy(S)
+    dp = DeviceBuffer(4 * max(n * K, 1))
+    it, delta = c_int32(), c_double()
+    rc = lib().gbo_ppr_forward(AT.ref, do.ptr, K, alpha, ds.ptr, dp.ptr, tol, max_iter, byref(it), byref(delta), None)
+    if rc != GBO_OK and (strict or rc != -7):
+        _check(rc)
+    return dp.to_numpy(np.float32, n * K).reshape(n, K), it.value, delta.value
+
+
+def ppr_adjoint(A: BsrMatrix, AT: BsrMatrix, outdeg, S, P, W, alpha: float = 0.85, tol: float = 1e-7,
+                max_iter: int = 1000, strict: bool = False):
+    """Gradients of L = sum(W * P) with respect to S and alpha:
+    (grad_S[n][K] float32, grad_alpha float, iterations)."""
+    S = np.ascontiguousarray(S, np.float32)
+    n, K = S.shape
+    do = DeviceBuffer.from_numpy(np.asarray(outdeg, np.int32))
+    ds = DeviceBuffer.from_numpy(S)
+    dp = DeviceBuffer.from_numpy(np.ascontiguousarray(P, np.float32))
+    dw = DeviceBuffer.from_numpy(np.ascontiguousarray(W, np.float32))
+    dg = DeviceBuffer(4 * max(n * K, 1))
+    ga, it = c_double(), c_int32()
+    rc = lib().gbo_ppr_adjoint(A.ref, AT.ref, do.ptr, K, alpha, ds.ptr, dp.ptr, dw.ptr, dg.ptr, byref(ga), tol,
+                               max_iter, byref(it), None)
+    if rc != GBO_OK and (strict or rc != -7):
+        _check(rc)
+    return dg.to_numpy(np.float32, n * K).reshape(n, K), ga.value, it.value

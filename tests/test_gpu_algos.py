@@ -152,3 +152,58 @@ def test_bfs_multisource(gbo, po, graph_cases, bs):
                 want = po.bfs(g.nrows, g.rowptr, g.colind, int(src[s]))
                 assert np.array_equal(lv[s], want), (name, k, s)
             assert nl == int(lv.max()) + 1, (name, k)
+
+
+def _ppr_case(gbo, seed, n_seeds=20, K=8):
+    g = gbo.gen_rmat(9, 8, seed=seed)
+    rng = np.random.default_rng(seed)
+    S = np.zeros((g.nrows, K), np.float64)
+    for k in range(K):
+        idx = rng.choice(g.nrows, n_seeds, replace=False)
+        S[idx, k] = rng.dirichlet(np.full(n_seeds, 0.1))
+    W = rng.standard_normal((g.nrows, K))
+    return g, S, W
+
+
+@pytest.mark.parametrize("bs", (8, 16, 32))
+def test_ppr_forward_adjoint(gbo, bs):
+    """GPU PPR forward (FP32) within 1e-6 L1 per column of the FP64 fixed
+    point; adjoint gradients within 1e-5 relative of the FP64 gradient."""
+    import ppr_oracle as pp
+    g, S, W = _ppr_case(gbo, 3)
+    A = gbo.BsrMatrix.from_csr(g, bs)
+    deg = g.outdeg()
+    P, it, delta = gbo.ppr_forward(A, deg, S, 0.85, 1e-8, 1000)
+    T = pp.transition(g.nrows, g.rowptr, g.colind)
+    want = pp.forward(T, S, 0.85)
+    assert np.abs(P - want).sum(0).max() <= 1e-6, (it, delta)
+    gS, ga, _ = gbo.ppr_adjoint(A, A, deg, S, P, W, 0.85, 1e-9, 1000)
+    wS, wa = pp.adjoint(T, S, W, 0.85)
+    assert np.abs(gS - wS).max() <= 1e-5 * np.abs(wS).max()
+    assert abs(ga - wa) <= 1e-5 * abs(wa)
+    # bitwise reproducible
+    P2, _, _ = gbo.ppr_forward(A, deg, S, 0.85, 1e-8, 1000)
+    assert np.array_equal(P.view(np.uint32), P2.view(np.uint32))
+
+
+def test_ppr_autograd(gbo):
+    """torch.autograd.Function: backward equals the FP64 adjoint and central
+    finite differences (h = 1e-3, FP64 forward) of the loss."""
+    torch = pytest.importorskip("torch")
+    import gbo_torch
+    import ppr_oracle as pp
+    g, S, W = _ppr_case(gbo, 5, K=4)
+    A = gbo.BsrMatrix.from_csr(g, 16)
+    St = torch.tensor(S, dtype=torch.float32, requires_grad=True)
+    at = torch.tensor(0.85, dtype=torch.float32, requires_grad=True)
+    P = gbo_torch.personalized_pagerank(St, at, A, A, g.outdeg(), 1e-9, 2000)
+    loss = (P * torch.tensor(W, dtype=torch.float32)).sum()
+    loss.backward()
+    T = pp.transition(g.nrows, g.rowptr, g.colind)
+    wS, wa = pp.adjoint(T, S, W, 0.85)
+    h = 1e-3
+    L = lambda a_: float((W * pp.forward(T, S, a_)).sum())
+    fd_a = (L(0.85 + h) - L(0.85 - h)) / (2 * h)
+    assert abs(at.grad.item() - wa) <= 1e-5 * abs(wa)
+    assert abs(at.grad.item() - fd_a) <= 1e-4 * abs(fd_a)
+    assert np.abs(St.grad.numpy() - wS).max() <= 1e-5 * np.abs(wS).max()

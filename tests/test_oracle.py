@@ -208,3 +208,28 @@ def test_count_sample_rows(po):
     full = po.count(16, 1000, brp, bci, tw, xb)
     part = po.count(16, 1000, brp, bci, tw, xb, 10, 40)
     assert np.array_equal(part[160:640], full[160:640]) and not part[:160].any()
+
+
+def test_ppr_oracle_finite_differences():
+    """FP64 adjoint formulas vs central differences (h = 1e-3) of the forward."""
+    import ppr_oracle as pp
+    rp, ci = random_sym(60, 150, 12)
+    rp = rp.copy()
+    P = pp.transition(60, rp, ci)
+    rng = np.random.default_rng(1)
+    S = rng.dirichlet(np.full(60, 0.3), size=3).T  # [n][K]
+    W = rng.standard_normal((60, 3))
+    alpha, h = 0.85, 1e-3
+    gS, ga = pp.adjoint(P, S, W, alpha)
+    L = lambda S_, a_: float((W * pp.forward(P, S_, a_)).sum())
+    fd_a = (L(S, alpha + h) - L(S, alpha - h)) / (2 * h)
+    assert abs(fd_a - ga) <= 1e-6 * max(1.0, abs(ga))
+    for (i, k) in ((0, 0), (7, 1), (33, 2)):
+        Sp, Sm = S.copy(), S.copy()
+        Sp[i, k] += h
+        Sm[i, k] -= h
+        fd = (L(Sp, alpha) - L(Sm, alpha)) / (2 * h)
+        assert abs(fd - gS[i, k]) <= 1e-8 + 1e-6 * abs(fd)
+    # p is a fixed point of the PageRank update
+    p = pp.forward(P, S, alpha)
+    assert np.abs(p - (alpha * P.T @ p + (1 - alpha) * S)).max() < 1e-12
