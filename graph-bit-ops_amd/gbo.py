@@ -124,6 +124,11 @@ def _declare(L):
         "gbo_cc": ([POINTER(BsrStruct), P, i32p, P], c_int),
         "gbo_tc": ([c_int, c_int32, P, P, POINTER(c_uint64), P], c_int),
         "gbo_bfs_multisource": ([POINTER(BsrStruct), P, c_int32, P, i32p, P], c_int),
+        "gbo_batch_create": ([c_int, c_int32, P, P, P, POINTER(P), P], c_int),
+        "gbo_batch_offsets": ([P, P], c_int),
+        "gbo_batch_info": ([P, i32p, i64p, i64p], c_int),
+        "gbo_batch_bfs_cc": ([P, P, P, P, c_int, P], c_int),
+        "gbo_batch_free": ([P], None),
         "gbo_ppr_forward": ([POINTER(BsrStruct), P, c_int32, c_float, P, P, c_double, c_int32, i32p,
                              POINTER(c_double), P], c_int),
         "gbo_ppr_adjoint": ([POINTER(BsrStruct), POINTER(BsrStruct), P, c_int32, c_float, P, P, P, P,
@@ -595,3 +600,64 @@ def ppr_adjoint(A: BsrMatrix, AT: BsrMatrix, outdeg, S, P, W, alpha: float = 0.8
     if rc != GBO_OK and (strict or rc != -7):
         _check(rc)
     return dg.to_numpy(np.float32, n * K).reshape(n, K), ga.value, it.value
+
+
+# ------------------------------------------------------------ graph batch
+class GraphBatch:
+    """Many graphs packed side by side; BFS + CC of all of them in one
+    persistent (optionally hipGraph-captured) launch."""
+
+    def __init__(self, graphs, bs: int):
+        L = lib()
+        G = len(graphs)
+        self.graphs = graphs
+        nrows = np.array([g.nrows for g in graphs], np.int32)
+        rps = (c_void_p * G)(*[g.rowptr.ctypes.data for g in graphs])
+        cis = (c_void_p * G)(*[g.colind.ctypes.data if g.nnz else 0 for g in graphs])
+        h = c_void_p()
+        _check(L.gbo_batch_create(bs, G, nrows.ctypes.data, ctypes.cast(rps, c_void_p), ctypes.cast(cis, c_void_p),
+                                  byref(h), None))
+        self.h = h.value
+        self.offsets = np.zeros(G + 1, np.int64)
+        _check(L.gbo_batch_offsets(self.h, self.offsets.ctypes.data))
+        bsv, nb, nv = c_int32(), c_int64(), c_int64()
+        _check(L.gbo_batch_info(self.h, byref(bsv), byref(nb), byref(nv)))
+        self.bs, self.nblocks, self.nvertices_padded = bsv.value, nb.value, nv.value
+        total = int(self.offsets[-1])
+        self.d_level = DeviceBuffer(4 * max(total, 1))
+        self.d_label = DeviceBuffer(4 * max(total, 1))
+
+    def run(self, sources, use_graph: bool = True):
+        src = np.ascontiguousarray(sources, np.int32)
+        _check(lib().gbo_batch_bfs_cc(self.h, src.ctypes.data, self.d_level.ptr, self.d_label.ptr, int(use_graph), None))
+
+    def results(self):
+        total = int(self.offsets[-1])
+        lv = self.d_level.to_numpy(np.int32, total)
+        lb = self.d_label.to_numpy(np.int32, total)
+        o = self.offsets
+        return [lv[o[i]:o[i + 1]] for i in range(len(o) - 1)], [lb[o[i]:o[i + 1]] for i in range(len(o) - 1)]
+
+    def free(self):
+        if getattr(self, "h", None):
+            lib().gbo_batch_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def gen_er_batch(count: int = 512, n_lo: int = 1000, n_hi: int = 65536, d_lo: float = 8.0, d_hi: float = 32.0,
+                 seed: int = 4):
+    """Erdos-Renyi graphs with n log-uniform in [n_lo, n_hi] and mean degree
+    uniform in [d_lo, d_hi] (G(n, m) form, m = n * d / 2 draws)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        n = int(round(np.exp(rng.uniform(np.log(n_lo), np.log(n_hi)))))
+        d = rng.uniform(d_lo, d_hi)
+        out.append(gen_er_m(n, int(n * d / 2), seed * 1_000_003 + i))
+    return out

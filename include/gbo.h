@@ -290,6 +290,25 @@ int gbo_tc(int bs, int32_t nrows, const int32_t* h_rowptr, const int32_t* h_coli
 int gbo_bfs_multisource(const gbo_bsr* AT, const int32_t* h_sources, int32_t nsrc, int32_t* d_level,
                         int32_t* h_nlevels, gbo_stream_t stream);
 
+/* Many-graph batch: G independent square graphs packed side by side into one
+ * block-diagonal bit-BSR (each graph's first vertex aligned to bs).  One
+ * persistent launch -- captured in a hipGraph when use_graph != 0 and
+ * replayed by later calls with the same output buffers -- runs BFS from
+ * h_sources[g] and FastSV connected components on every graph, each graph
+ * on one workgroup with no host synchronisation between iterations.
+ * Outputs are concatenated per graph (offsets from gbo_batch_offsets):
+ * d_level as gbo_bfs, d_label = smallest graph-local vertex id of the
+ * component, identical to per-graph gbo_bfs / gbo_cc runs.  No reference
+ * counterpart (the reference runs one graph per process). */
+typedef struct gbo_batch gbo_batch;
+int gbo_batch_create(int bs, int32_t ngraphs, const int32_t* h_nrows, const int32_t* const* h_rowptr,
+                     const int32_t* const* h_colind, gbo_batch** out, gbo_stream_t stream);
+int gbo_batch_offsets(const gbo_batch* b, int64_t* h_offsets /* ngraphs+1 */);
+int gbo_batch_info(const gbo_batch* b, int32_t* bs, int64_t* nblocks, int64_t* nvertices_padded);
+int gbo_batch_bfs_cc(gbo_batch* b, const int32_t* h_sources, int32_t* d_level, int32_t* d_label, int use_graph,
+                     gbo_stream_t stream);
+void gbo_batch_free(gbo_batch* b);
+
 /* -------------------------------------------------- differentiation (PPR) --
  * Personalized PageRank over K <= 64 vectors stored vertex-major
  * ([n][K] float):  p_k = alpha * P^T p_k + (1 - alpha) * s_k,  P = D^-1 A,

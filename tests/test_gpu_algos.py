@@ -207,3 +207,26 @@ def test_ppr_autograd(gbo):
     assert abs(at.grad.item() - wa) <= 1e-5 * abs(wa)
     assert abs(at.grad.item() - fd_a) <= 1e-4 * abs(fd_a)
     assert np.abs(St.grad.numpy() - wS).max() <= 1e-5 * np.abs(wS).max()
+
+
+@pytest.mark.parametrize("bs", BS)
+def test_graph_batch(gbo, po, bs):
+    """One persistent launch (direct and hipGraph-captured, replayed) gives
+    the same BFS levels and CC labels as per-graph plain runs and the oracle."""
+    graphs = gbo.gen_er_batch(count=24, n_lo=50, n_hi=3000, d_lo=0.5, d_hi=6.0, seed=4)
+    graphs.append(gbo.gen_rmat(10, 4, seed=6))
+    rng = np.random.default_rng(bs)
+    src = np.array([rng.integers(0, g.nrows) for g in graphs], np.int32)
+    B = gbo.GraphBatch(graphs, bs)
+    for use_graph in (False, True, True):
+        B.run(src, use_graph)
+        levels, labels = B.results()
+        for i, g in enumerate(graphs):
+            assert np.array_equal(levels[i], po.bfs(g.nrows, g.rowptr, g.colind, int(src[i]))), (i, use_graph)
+            assert np.array_equal(labels[i], po.cc(g.nrows, g.rowptr, g.colind)), (i, use_graph)
+    # plain per-graph runs agree too
+    for i in (0, len(graphs) - 1):
+        A = gbo.BsrMatrix.from_csr(graphs[i], bs)
+        lv, _ = gbo.bfs(A, int(src[i]))
+        lb, _ = gbo.cc(A)
+        assert np.array_equal(lv, levels[i]) and np.array_equal(lb, labels[i])
