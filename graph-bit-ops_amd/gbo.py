@@ -36,6 +36,12 @@ WORD_DTYPE = {8: np.uint8, 16: np.uint16, 32: np.uint32, 64: np.uint64}
 INF_U32 = 0xFFFFFFFF
 
 
+class Tuning(ctypes.Structure):
+    """Mirror of ``gbo_tuning``."""
+
+    _fields_ = [("bs", c_int32), ("chunk", c_int32), ("variant", c_int32), ("ms", c_float)]
+
+
 class GboError(RuntimeError):
     """A negative GBO_E* status returned through the C ABI."""
 
@@ -124,6 +130,7 @@ def _declare(L):
         "gbo_cc": ([POINTER(BsrStruct), P, i32p, P], c_int),
         "gbo_tc": ([c_int, c_int32, P, P, POINTER(c_uint64), P], c_int),
         "gbo_bfs_multisource": ([POINTER(BsrStruct), P, c_int32, P, i32p, P], c_int),
+        "gbo_autotune": ([P, P, c_int32, c_int32, c_int, c_int, c_int, POINTER(Tuning), P], c_int),
         "gbo_batch_create": ([c_int, c_int32, P, P, P, POINTER(P), P], c_int),
         "gbo_batch_offsets": ([P, P], c_int),
         "gbo_batch_info": ([P, i32p, i64p, i64p], c_int),
@@ -661,3 +668,12 @@ def gen_er_batch(count: int = 512, n_lo: int = 1000, n_hi: int = 65536, d_lo: fl
         d = rng.uniform(d_lo, d_hi)
         out.append(gen_er_m(n, int(n * d / 2), seed * 1_000_003 + i))
     return out
+
+
+def autotune(d_rowptr: DeviceBuffer, d_colind: DeviceBuffer, nrows: int, ncols: int, op: int = 0,
+             block_sizes=(8, 16, 32, 64), reps: int = 3) -> Tuning:
+    """Fastest (bs, chunk, variant) for a bsrbmv sweep of this device CSR."""
+    mask = sum(1 << {8: 0, 16: 1, 32: 2, 64: 3}[b] for b in block_sizes)
+    t = Tuning()
+    _check(lib().gbo_autotune(d_rowptr.ptr, d_colind.ptr, nrows, ncols, op, mask, reps, byref(t), None))
+    return t

@@ -290,6 +290,19 @@ int gbo_tc(int bs, int32_t nrows, const int32_t* h_rowptr, const int32_t* h_coli
 int gbo_bfs_multisource(const gbo_bsr* AT, const int32_t* h_sources, int32_t nsrc, int32_t* d_level,
                         int32_t* h_nlevels, gbo_stream_t stream);
 
+/* Autotuner: packs the device CSR at every block size in bs_mask (bit k =
+ * bs 8<<k) and times `reps` sweeps of op 0 (gbo_bsrbmv_count) or op 1
+ * (gbo_bsrbmv_bfs_step) for each kernel variant and three chunk sizes;
+ * returns the fastest (block size, chunk, variant, ms).  Cached per graph
+ * signature for the life of the process.  No reference counterpart (the
+ * reference fixes the block size at build time, bsrbmv/Makefile:57-67). */
+typedef struct gbo_tuning {
+  int32_t bs, chunk, variant;
+  float ms;
+} gbo_tuning;
+int gbo_autotune(const int32_t* d_rowptr, const int32_t* d_colind, int32_t nrows, int32_t ncols, int op,
+                 int bs_mask, int reps, gbo_tuning* out, gbo_stream_t stream);
+
 /* Many-graph batch: G independent square graphs packed side by side into one
  * block-diagonal bit-BSR (each graph's first vertex aligned to bs).  One
  * persistent launch -- captured in a hipGraph when use_graph != 0 and

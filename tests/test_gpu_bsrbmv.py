@@ -111,3 +111,21 @@ def test_min_i32(gbo, po, graph_cases, bs, chunk):
         A, (rp, ci, tw) = packed(gbo, g, bs, chunk)
         x = rng.integers(-2**31, 2**31 - 1, g.ncols).astype(np.int32)
         assert np.array_equal(gbo.bsrbmv_min_i32(A, x), po.min_i32(bs, g.nrows, rp, ci, tw, x)), name
+
+
+def test_autotune(gbo, po):
+    """The autotuner returns a valid configuration whose results are the
+    same as the default's (variants and chunk sizes never change results)."""
+    g = gbo.gen_rmat(12, 16, seed=1)
+    d_rp = gbo.DeviceBuffer.from_numpy(g.rowptr)
+    d_ci = gbo.DeviceBuffer.from_numpy(g.colind)
+    t = gbo.autotune(d_rp, d_ci, g.nrows, g.ncols, 0)
+    assert t.bs in (8, 16, 32, 64) and t.chunk > 0 and t.ms > 0
+    t2 = gbo.autotune(d_rp, d_ci, g.nrows, g.ncols, 0)  # cached
+    assert (t2.bs, t2.chunk, t2.variant) == (t.bs, t.chunk, t.variant)
+    A = gbo.BsrMatrix.from_csr(g, t.bs, plan=False)
+    A.plan(t.chunk, t.variant)
+    x = (np.arange(g.ncols) % 3 == 0).astype(np.float32)
+    xb = gbo.pack_vec(t.bs, x)
+    rp, ci, tw = A.host_arrays()
+    assert np.array_equal(gbo.bsrbmv_count(A, xb), po.count(t.bs, g.nrows, rp, ci, tw, xb))
