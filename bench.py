@@ -95,6 +95,92 @@ def copy_ceiling(gbo, timer, nbytes=1 << 30, reps=10):
     return round(2 * nbytes / (ms * 1e-3) / 1e9, 1)
 
 
+def bench_multisource(gbo, A, g, nsrc=256, nplain=32, seed=3):
+    """C4a: 256-source batched BFS vs plain per-source BFS launches (same GPU).
+    Throughput = sources * nnz / time.  The plain rate is measured on the
+    first `nplain` sources and scaled per source."""
+    rng = np.random.default_rng(seed)
+    src = rng.integers(0, g.nrows, nsrc).astype(np.int32)
+    out = gbo.DeviceBuffer(4 * nsrc * g.nrows)
+    gbo.bfs_multisource(A, src[:64], out)  # warm-up
+    gbo.synchronize()
+    t = time.perf_counter()
+    nl = gbo.bfs_multisource(A, src, out)
+    gbo.synchronize()
+    tb = time.perf_counter() - t
+    lv_b = out.to_numpy(np.int32, nsrc * g.nrows).reshape(nsrc, g.nrows)
+    exact = True
+    t = time.perf_counter()
+    for i in range(nplain):
+        lv, _ = gbo.bfs(A, int(src[i]))
+        exact &= bool(np.array_equal(lv, lv_b[i]))
+    tp = (time.perf_counter() - t) / nplain
+    out.free()
+    rate_b = nsrc * g.nnz / tb / 1e9
+    rate_p = g.nnz / tp / 1e9
+    return {"sources": nsrc, "levels": nl, "batched_ms": round(tb * 1e3, 2), "plain_ms_per_source": round(tp * 1e3, 3),
+            "batched_GTEPS": round(rate_b, 2), "plain_GTEPS": round(rate_p, 3), "speedup": round(rate_b / rate_p, 2),
+            "bit_exact_vs_plain": exact, "plain_sample": nplain, "bs": A.bs}
+
+
+def bench_graph_batch(gbo, count=512, bs=8, seed=4):
+    """C4b: `count` ER graphs (n log-uniform in [1k, 64k], mean degree in
+    [8, 32]) -- BFS + CC of all of them in one persistent hipGraph launch vs
+    per-graph plain gbo_bfs + gbo_cc."""
+    t = time.perf_counter()
+    graphs = gbo.gen_er_batch(count=count, seed=seed)
+    gen_s = time.perf_counter() - t
+    nnz = sum(g.nnz for g in graphs)
+    rng = np.random.default_rng(seed)
+    src = np.array([rng.integers(0, g.nrows) for g in graphs], np.int32)
+    t = time.perf_counter()
+    B = gbo.GraphBatch(graphs, bs)
+    pack_s = time.perf_counter() - t
+    B.run(src, use_graph=True)  # capture + first replay
+    t = time.perf_counter()
+    reps = 3
+    for _ in range(reps):
+        B.run(src, use_graph=True)
+    tb = (time.perf_counter() - t) / reps
+    levels, labels = B.results()
+    exact = True
+    tp = 0.0
+    for i, g in enumerate(graphs):
+        A = gbo.BsrMatrix.from_csr(g, bs)
+        gbo.synchronize()
+        t = time.perf_counter()
+        lv, _ = gbo.bfs(A, int(src[i]))
+        lb, _ = gbo.cc(A)
+        tp += time.perf_counter() - t
+        exact &= bool(np.array_equal(lv, levels[i]) and np.array_equal(lb, labels[i]))
+        A.free()
+    B.free()
+    return {"graphs": count, "vertices": int(sum(g.nrows for g in graphs)), "nnz": int(nnz), "bs": bs,
+            "gen_s": round(gen_s, 1), "pack_s": round(pack_s, 2), "batched_ms": round(tb * 1e3, 2),
+            "plain_ms": round(tp * 1e3, 1), "speedup": round(tp / tb, 2), "bit_exact_vs_plain": exact}
+
+
+def bench_ppr(gbo, A, g, K=64, nseeds=1000, seed=5, tol=1e-6, max_iter=200):
+    """C5: PPR forward and adjoint, 64 Dirichlet(0.1) personalization
+    vectors over 1000 seed vertices each; w ~ N(0, 1)."""
+    rng = np.random.default_rng(seed)
+    S = np.zeros((g.nrows, K), np.float32)
+    for k in range(K):
+        idx = rng.choice(g.nrows, nseeds, replace=False)
+        S[idx, k] = rng.dirichlet(np.full(nseeds, 0.1)).astype(np.float32)
+    W = rng.standard_normal((g.nrows, K)).astype(np.float32)
+    deg = g.outdeg()
+    t = time.perf_counter()
+    P, itf, df = gbo.ppr_forward(A, deg, S, 0.85, tol, max_iter)
+    tf = time.perf_counter() - t
+    t = time.perf_counter()
+    gS, ga, ita = gbo.ppr_adjoint(A, A, deg, S, P, W, 0.85, tol, max_iter)
+    ta = time.perf_counter() - t
+    return {"K": K, "bs": A.bs, "forward_ms": round(tf * 1e3, 1), "forward_iters": itf, "forward_delta": df,
+            "adjoint_ms": round(ta * 1e3, 1), "adjoint_iters": ita, "adjoint_over_forward": round(ta / tf, 3),
+            "ms_per_spmm_iter": round(tf * 1e3 / max(itf, 1), 2), "grad_alpha": ga}
+
+
 def cpu_baseline(gbo, A, xbits, n, target_s):
     """Oracle OpenMP sweep over a contiguous sample of block rows."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -139,6 +225,11 @@ def cpu_baseline(gbo, A, xbits, n, target_s):
     }
 
 
+def keep_ppr(gbo, d_rp, d_ci, n):
+    """The PPR SpMM runs at bs 8 (one 256-B row gather per set bit)."""
+    return gbo.BsrMatrix.from_device_csr(d_rp, d_ci, n, n, 8, 0)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -150,6 +241,7 @@ def main():
     ap.add_argument("--no-algos", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--variants", default="0,1,2", help="bs 8/16 kernel variants to time (best is kept)")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched BFS / graph batch / PPR lines")
     ap.add_argument("--profile", action="store_true",
                     help="short run for rocprofv3: only the count sweeps and the copy kernel")
     args = ap.parse_args()
@@ -172,7 +264,7 @@ def main():
     copy_gbs = copy_ceiling(gbo, timer)
     log(f"[bench] copy-kernel ceiling {copy_gbs} GB/s")
     if args.profile:
-        args.no_algos = args.no_cpu = True
+        args.no_algos = args.no_cpu = args.no_batch = True
     x = (np.random.default_rng(0).random(n) < 0.5).astype(np.float32)
     dxf = gbo.DeviceBuffer.from_numpy(x)
     dy = gbo.DeviceBuffer(4 * n)
@@ -234,6 +326,18 @@ def main():
     best = min(per_bs.values(), key=lambda r: r["ms_per_sweep"])
     bs = best["bs"]
     A, dx = keep[bs]
+    extra = {}
+    if not args.no_batch:
+        for name, fn in (("multisource_bfs", lambda: bench_multisource(gbo, A, g)),
+                         ("graph_batch", lambda: bench_graph_batch(gbo)),
+                         ("ppr", lambda: bench_ppr(gbo, keep_ppr(gbo, d_rp, d_ci, n), g))):
+            try:
+                t = time.time()
+                extra[name] = fn()
+                log(f"[bench] {name} ({time.time() - t:.1f}s): {extra[name]}")
+            except Exception as e:  # a failing side line must not hide the headline
+                extra[name] = {"error": str(e)}
+                log(f"[bench] {name} failed: {e}")
     cpu = None
     if not args.no_cpu:
         xb = gbo.pack_vec(bs, x)
@@ -250,6 +354,7 @@ def main():
                      "frac": best["frac_hbm"], "traffic": None, "copy_ceiling_GBs": copy_gbs},
         "cpu_baseline": cpu,
         "per_block_size": per_bs,
+        "batched": extra,
     }
     print(json.dumps(out), flush=True)
 
