@@ -32,7 +32,7 @@ METRIC = "bsrbmv GTEPS + HBM roofline frac, R-MAT s22; batched/adjoint vs plain 
 
 
 def gbo_default_chunk(bs):
-    return {8: 4096, 16: 2048, 32: 512, 64: 256}[bs]
+    return {8: 512, 16: 256, 32: 256, 64: 128}[bs]
 
 
 def log(*a):

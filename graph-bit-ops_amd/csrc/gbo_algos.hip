@@ -363,7 +363,8 @@ int gbo_pagerank(const gbo_bsr* AT, const int32_t* d_outdeg, float alpha, double
   if (!(alpha >= 0.0f && alpha < 1.0f) || max_iter < 1) return set_error(GBO_EINVAL, "gbo_pagerank: bad alpha/max_iter");
   if (n == 0) return GBO_OK;
   hipStream_t st = stream;
-  float *x = nullptr, *y = nullptr, *partial = nullptr;
+  float *x = nullptr, *y = nullptr;
+  double* partial = nullptr;
   double* dpart = nullptr;
   const int nparts = 1024;
   GBO_HIP_CHECK(hipMallocAsync((void**)&x, sizeof(float) * (size_t)n * 2, st));
@@ -372,7 +373,7 @@ int gbo_pagerank(const gbo_bsr* AT, const int32_t* d_outdeg, float alpha, double
   PlanView pv;
   rc = acquire_plan(AT, &pv, st);
   if (rc) return rc;
-  GBO_HIP_CHECK(hipMallocAsync((void**)&partial, sizeof(float) * 2 * (size_t)pv.nchunks * AT->bs, st));
+  GBO_HIP_CHECK(hipMallocAsync((void**)&partial, sizeof(double) * 2 * (size_t)pv.nchunks * AT->bs, st));
   rc = fill<float>(d_p, n, 1.0f / (float)n, st);
   SpmvArgs a{};
   a.rowptr = AT->rowptr;

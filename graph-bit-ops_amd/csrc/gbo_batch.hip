@@ -39,7 +39,7 @@ __global__ __launch_bounds__(256) void k_msbfs(const MsArgs a) {
   constexpr int G = kWave / BS;
   const int lane = lane_id();
   const int g = lane / BS, r = lane % BS;
-  const int c = (int)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave);
+  const int c = wave_index();
   if (c >= a.nchunks) return;
   const int64_t cb = (int64_t)c * a.chunk;
   const int64_t ce = min(cb + (int64_t)a.chunk, a.T);
@@ -67,7 +67,7 @@ __global__ __launch_bounds__(256) void k_msbfs(const MsArgs a) {
       const int j = __builtin_ctzll(work);
       work &= work - 1;
       const int32_t R = R0 + j;
-      const int64_t rs = __shfl(rsl, j), re = __shfl(rel, j);
+      const int64_t rs = __builtin_amdgcn_readlane(rsl, j), re = __builtin_amdgcn_readlane(rel, j);
       const int64_t s = rs > cb ? rs : cb;
       const int64_t te = re < ce ? re : ce;
       const bool full = (rs >= cb) && (re <= ce);

@@ -53,7 +53,7 @@ template <int BS>
 __global__ __launch_bounds__(256) void k_tc(const TcArgs a) {
   using W = word_t<BS>;
   const int lane = lane_id();
-  const int c = (int)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave);
+  const int c = wave_index();
   if (c >= a.nchunks) return;
   const int64_t cb = (int64_t)c * a.chunk;
   const int64_t ce = min(cb + (int64_t)a.chunk, a.T);
@@ -74,7 +74,7 @@ __global__ __launch_bounds__(256) void k_tc(const TcArgs a) {
     while (work) {
       const int j = __builtin_ctzll(work);
       work &= work - 1;
-      const int64_t rs = __shfl(rsl, j), re = __shfl(rel, j);
+      const int64_t rs = __builtin_amdgcn_readlane(rsl, j), re = __builtin_amdgcn_readlane(rel, j);
       const int64_t s = rs > cb ? rs : cb;
       const int64_t te = re < ce ? re : ce;
       for (int64_t tm = s; tm < te; ++tm) {  // mask tile (I, K)

@@ -18,15 +18,6 @@
 
 namespace gbo {
 
-// Neumaier's compensated addition: s + c carries the sum of all added terms
-// with the rounding error of each addition accumulated in c.
-__device__ __forceinline__ void kahan_add(float& s, float& c, float v) {
-  const float t = s + v;
-  if (fabsf(s) >= fabsf(v)) c += (s - t) + v;
-  else c += (v - t) + s;
-  s = t;
-}
-
 // Identity written for a block row with no tiles (lane-per-row, see k_spmv).
 template <int BS, int OP>
 __device__ __forceinline__ void write_empty_row(const SpmvArgs& a, int32_t R) {
@@ -55,7 +46,7 @@ __global__ __launch_bounds__(256) void k_spmv(const SpmvArgs a) {
   constexpr int G = kWave / BS;
   const int lane = lane_id();
   const int g = lane / BS, r = lane % BS;
-  const int c = (int)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave);
+  const int c = wave_index();
   if (c >= a.nchunks) return;  // wave-uniform
   const int64_t cb = (int64_t)c * a.chunk;
   const int64_t ce = min(cb + (int64_t)a.chunk, a.T);
@@ -82,7 +73,7 @@ __global__ __launch_bounds__(256) void k_spmv(const SpmvArgs a) {
       const int j = __builtin_ctzll(work);
       work &= work - 1;
       const int32_t R = R0 + j;
-      const int64_t rs = __shfl(rsl, j), re = __shfl(rel, j);
+      const int64_t rs = __builtin_amdgcn_readlane(rsl, j), re = __builtin_amdgcn_readlane(rel, j);
       const int64_t s = rs > cb ? rs : cb;
       const int64_t te = re < ce ? re : ce;
       const bool full = (rs >= cb) && (re <= ce);
@@ -126,13 +117,22 @@ __global__ __launch_bounds__(256) void k_spmv(const SpmvArgs a) {
           if (full && lane == 0) nx[R] = (W)0;  // every row of the block already visited
           continue;
         }
+        // Tile words and column indices of eight tiles are loaded together,
+        // then their frontier words: two dependent loads per eight tiles.
         int hit = 0;
-        for (int64_t t = s + g; t < te; t += G) {
-          const W f = fb[colind[t]];
-          if (f != 0 && !hit) {  // skip tiles whose frontier word is empty
-            if ((W)(tiles[t * BS + r] & f) != 0) hit = 1;
+        int64_t t = s + g;
+        for (; t + 7 * G < te; t += 8 * G) {
+          int32_t cc[8];
+          W ww[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            cc[u] = colind[t + u * G];
+            ww[u] = tiles[(t + u * G) * BS + r];
           }
+#pragma unroll
+          for (int u = 0; u < 8; ++u) hit |= (W)(ww[u] & fb[cc[u]]) != 0;
         }
+        for (; t < te; t += G) hit |= (W)(tiles[t * BS + r] & fb[colind[t]]) != 0;
         for (int off = BS; off < kWave; off <<= 1) hit |= __shfl_xor(hit, off);
         const uint64_t bal = __ballot(hit);
         const uint64_t word = brev_low<BS>(bal & low_mask<BS>()) & ~(uint64_t)vword & low_mask<BS>();
@@ -148,29 +148,49 @@ __global__ __launch_bounds__(256) void k_spmv(const SpmvArgs a) {
       } else if constexpr (OP == OP_MINPLUS) {
         const uint32_t* __restrict__ dist = (const uint32_t*)a.xval;
         uint32_t acc = kInfU32;
-        for (int64_t t = s + g; t < te; t += G) {
-          const int32_t col = colind[t];
-          if (a.active && !((a.active[col >> 6] >> (col & 63)) & 1ull)) continue;  // group-uniform
-          uint64_t wd = (uint64_t)tiles[t * BS + r];
-          const int pc = popcw(wd);
-          int rank0 = 0;
-          if (a.w) {
-            int incl = pc;
-            for (int off = 1; off < BS; off <<= 1) {
-              const int o = __shfl_up(incl, off, BS);
-              if (r >= off) incl += o;
+        // Four tiles per round: their words, column indices and weight
+        // ranks are formed first, then one set bit of each tile is relaxed
+        // per pass, so the distance gathers of a pass are independent.
+        for (int64_t t = s + g; t < te; t += 4 * G) {
+          int32_t cc[4];
+          uint64_t wd[4];
+          int rank[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const int64_t tu = t + u * G;
+            cc[u] = 0;
+            wd[u] = 0;
+            if (tu < te) {
+              cc[u] = colind[tu];
+              wd[u] = (uint64_t)tiles[tu * BS + r];
+              // inactive source block column: the tile is skipped (group-uniform)
+              if (a.active && !((a.active[cc[u] >> 6] >> (cc[u] & 63)) & 1ull)) wd[u] = 0;
             }
-            rank0 = incl - pc;
+            // index of this row's last set bit among the tile's weights:
+            // rows r' < r come first in tile bit order
+            const int pc = popcw(wd[u]);
+            int incl = pc;
+            if (a.w) {
+              for (int off = 1; off < BS; off <<= 1) {
+                const int o = __shfl_up(incl, off, BS);
+                if (r >= off) incl += o;
+              }
+            }
+            rank[u] = incl - 1;  // set bits are taken from the last column back
           }
-          int k = 0;
-          while (wd) {
-            const int p = __builtin_ctzll(wd);
-            const uint32_t dv = dist[(int64_t)col * BS + (BS - 1 - p)];
-            const uint32_t wt = a.w ? a.w[a.woff[t] + rank0 + (pc - 1 - k)] : 1u;
-            const uint32_t cand = (dv > kInfU32 - wt) ? kInfU32 : dv + wt;
-            acc = cand < acc ? cand : acc;
-            wd &= wd - 1;
-            ++k;
+          while (wd[0] | wd[1] | wd[2] | wd[3]) {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+              if (wd[u]) {
+                const int p = __builtin_ctzll(wd[u]);
+                const uint32_t dv = dist[(int64_t)cc[u] * BS + (BS - 1 - p)];
+                const uint32_t wt = a.w ? a.w[a.woff[t + u * G] + rank[u]] : 1u;
+                const uint32_t cand = (dv > kInfU32 - wt) ? kInfU32 : dv + wt;
+                acc = cand < acc ? cand : acc;
+                wd[u] &= wd[u] - 1;
+                --rank[u];
+              }
+            }
           }
         }
         for (int off = BS; off < kWave; off <<= 1) {
@@ -187,44 +207,68 @@ __global__ __launch_bounds__(256) void k_spmv(const SpmvArgs a) {
           }
         }
       } else if constexpr (OP == OP_SUM) {
-        // Compensated (Neumaier) FP32 summation: the error of a row sum stays
-        // within a few ulps of the result whatever the row length (a hub
-        // row adds >10^5 terms).
+        // FP64 accumulation of the FP32 terms: a row sum (a hub row adds
+        // >10^5 terms) keeps far more than FP32 resolution, in a fixed order
+        // (lane order, then the butterfly), so results are reproducible.
+        // Four tiles per round; one set bit of each per pass, so the x
+        // gathers of a pass are independent.
         const float* __restrict__ x = (const float*)a.xval;
-        float acc = 0.0f, comp = 0.0f;
-        for (int64_t t = s + g; t < te; t += G) {
-          const int32_t col = colind[t];
-          uint64_t wd = (uint64_t)tiles[t * BS + r];
-          while (wd) {
-            const int p = __builtin_ctzll(wd);
-            kahan_add(acc, comp, x[(int64_t)col * BS + (BS - 1 - p)]);
-            wd &= wd - 1;
+        double acc = 0.0;
+        for (int64_t t = s + g; t < te; t += 4 * G) {
+          int32_t cc[4];
+          uint64_t wd[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const int64_t tu = t + u * G;
+            cc[u] = tu < te ? colind[tu] : 0;
+            wd[u] = tu < te ? (uint64_t)tiles[tu * BS + r] : 0ull;
+          }
+          while (wd[0] | wd[1] | wd[2] | wd[3]) {
+            float v[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+              v[u] = 0.0f;
+              if (wd[u]) {
+                const int p = __builtin_ctzll(wd[u]);
+                v[u] = x[(int64_t)cc[u] * BS + (BS - 1 - p)];
+                wd[u] &= wd[u] - 1;
+              }
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) acc += (double)v[u];
           }
         }
-        for (int off = BS; off < kWave; off <<= 1) {
-          const float os = __shfl_xor(acc, off), oc = __shfl_xor(comp, off);
-          kahan_add(acc, comp, os);
-          comp += oc;
-        }
+        for (int off = BS; off < kWave; off <<= 1) acc += __shfl_xor(acc, off);
         if (g == 0 && i < a.nrows) {
           if (full) {
-            ((float*)a.out)[i] = a.scale * (acc + comp) + a.bias;
+            ((float*)a.out)[i] = (float)((double)a.scale * acc + (double)a.bias);
           } else {
             const int slot = (rs < cb) ? 0 : 1;
-            a.partial[((int64_t)c * 2 + slot) * BS + r] = acc + comp;
+            a.partial[((int64_t)c * 2 + slot) * BS + r] = acc;
           }
         }
       } else if constexpr (OP == OP_MIN) {
         const int32_t* __restrict__ x = (const int32_t*)a.xval;
         int32_t acc = INT32_MAX;
-        for (int64_t t = s + g; t < te; t += G) {
-          const int32_t col = colind[t];
-          uint64_t wd = (uint64_t)tiles[t * BS + r];
-          while (wd) {
-            const int p = __builtin_ctzll(wd);
-            const int32_t v = x[(int64_t)col * BS + (BS - 1 - p)];
-            acc = v < acc ? v : acc;
-            wd &= wd - 1;
+        for (int64_t t = s + g; t < te; t += 4 * G) {
+          int32_t cc[4];
+          uint64_t wd[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const int64_t tu = t + u * G;
+            cc[u] = tu < te ? colind[tu] : 0;
+            wd[u] = tu < te ? (uint64_t)tiles[tu * BS + r] : 0ull;
+          }
+          while (wd[0] | wd[1] | wd[2] | wd[3]) {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+              if (wd[u]) {
+                const int p = __builtin_ctzll(wd[u]);
+                const int32_t v = x[(int64_t)cc[u] * BS + (BS - 1 - p)];
+                acc = v < acc ? v : acc;
+                wd[u] &= wd[u] - 1;
+              }
+            }
           }
         }
         for (int off = BS; off < kWave; off <<= 1) {
@@ -271,6 +315,30 @@ __device__ __forceinline__ void popc_fields(const uint64_t* w, uint64_t x, uint6
   }
 }
 
+// Butterfly reduce-scatter of per-lane row partials v[0..BS) (v is
+// overwritten): every lane of group l / (64/BS) gets the wave total of row
+// l / (64/BS).
+template <int BS>
+__device__ __forceinline__ uint32_t reduce_scatter_rows(uint32_t* v) {
+  constexpr int LPR = kWave / BS;
+  const int lane = lane_id();
+#pragma unroll
+  for (int lvl = 0, nv = BS; nv > 1; ++lvl, nv >>= 1) {
+    const int step = 32 >> lvl;
+    const bool upper = (lane & step) != 0;
+#pragma unroll
+    for (int k = 0; k < nv / 2; ++k) {
+      const uint32_t send = upper ? v[k] : v[k + nv / 2];
+      const uint32_t keep = upper ? v[k + nv / 2] : v[k];
+      v[k] = keep + (uint32_t)__shfl_xor((int)send, step);
+    }
+  }
+  uint32_t tot = v[0];
+#pragma unroll
+  for (int off = LPR / 2; off > 0; off >>= 1) tot += (uint32_t)__shfl_xor((int)tot, off);
+  return tot;
+}
+
 template <int BS>
 __device__ __forceinline__ void load_tile(const void* tiles, int64_t t, uint64_t* w) {
   if constexpr (BS == 8) {
@@ -293,7 +361,7 @@ __global__ __launch_bounds__(256) void k_spmv_lpt(const SpmvArgs a) {
   constexpr int NA = (BS == 8) ? 2 : 4;  // packed 16-bit-field accumulators
   constexpr int LPR = kWave / BS;        // lanes per row after the reduce-scatter
   const int lane = lane_id();
-  const int c = (int)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave);
+  const int c = wave_index();
   if (c >= a.nchunks) return;
   const int64_t cb = (int64_t)c * a.chunk;
   const int64_t ce = min(cb + (int64_t)a.chunk, a.T);
@@ -317,7 +385,7 @@ __global__ __launch_bounds__(256) void k_spmv_lpt(const SpmvArgs a) {
       const int j = __builtin_ctzll(work);
       work &= work - 1;
       const int32_t R = R0 + j;
-      const int64_t rs = __shfl(rsl, j), re = __shfl(rel, j);
+      const int64_t rs = __builtin_amdgcn_readlane(rsl, j), re = __builtin_amdgcn_readlane(rel, j);
       const int64_t s = rs > cb ? rs : cb;
       const int64_t te = re < ce ? re : ce;
       const bool full = (rs >= cb) && (re <= ce);
@@ -379,22 +447,7 @@ __global__ __launch_bounds__(256) void k_spmv_lpt(const SpmvArgs a) {
           add_tile(w0, xb[colind[t]]);
         }
         flush();
-        // butterfly reduce-scatter: after the loop lane l holds a partial
-        // of row l / LPR; then sum the LPR lanes of that row
-#pragma unroll
-        for (int lvl = 0, nv = BS; nv > 1; ++lvl, nv >>= 1) {
-          const int step = 32 >> lvl;
-          const bool upper = (lane & step) != 0;
-#pragma unroll
-          for (int k = 0; k < nv / 2; ++k) {
-            const uint32_t send = upper ? v[k] : v[k + nv / 2];
-            const uint32_t keep = upper ? v[k + nv / 2] : v[k];
-            v[k] = keep + (uint32_t)__shfl_xor((int)send, step);
-          }
-        }
-        uint32_t tot = v[0];
-#pragma unroll
-        for (int off = LPR / 2; off > 0; off >>= 1) tot += (uint32_t)__shfl_xor((int)tot, off);
+        const uint32_t tot = reduce_scatter_rows<BS>(v);
         const int row = lane / LPR;
         const int64_t i = (int64_t)R * BS + row;
         if ((lane % LPR) == 0 && i < a.nrows) {
@@ -418,15 +471,29 @@ __global__ __launch_bounds__(256) void k_spmv_lpt(const SpmvArgs a) {
         uint64_t hit[NW];
 #pragma unroll
         for (int k = 0; k < NW; ++k) hit[k] = 0;
-        for (int64_t t = s + lane; t < te; t += kWave) {
-          const uint64_t f = xb[colind[t]];
-          if (f) {  // tiles whose frontier word is empty are not read
-            uint64_t w[NW];
-            load_tile<BS>(a.tiles, t, w);
-            const uint64_t xr = (BS == 8) ? f * 0x0101010101010101ull : f * 0x0001000100010001ull;
+        // tiles and column indices of four steps are loaded before their
+        // frontier words
+        auto or_tile = [&](const uint64_t* w, uint64_t f) {
+          const uint64_t xr = (BS == 8) ? f * 0x0101010101010101ull : f * 0x0001000100010001ull;
 #pragma unroll
-            for (int k = 0; k < NW; ++k) hit[k] |= w[k] & xr;
+          for (int k = 0; k < NW; ++k) hit[k] |= w[k] & xr;
+        };
+        int64_t t = s + lane;
+        for (; t + 3 * kWave < te; t += 4 * kWave) {
+          uint64_t w[4][NW];
+          int32_t cc[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            cc[u] = colind[t + u * kWave];
+            load_tile<BS>(a.tiles, t + u * kWave, w[u]);
           }
+#pragma unroll
+          for (int u = 0; u < 4; ++u) or_tile(w[u], (uint64_t)xb[cc[u]]);
+        }
+        for (; t < te; t += kWave) {
+          uint64_t w[NW];
+          load_tile<BS>(a.tiles, t, w);
+          or_tile(w, (uint64_t)xb[colind[t]]);
         }
 #pragma unroll
         for (int k = 0; k < NW; ++k)
@@ -458,184 +525,204 @@ __global__ __launch_bounds__(256) void k_spmv_lpt(const SpmvArgs a) {
   }
 }
 
-// Row outputs that the segmented kernel never writes: empty rows get the
-// identity; rows cut by a chunk boundary are initialised for the atomics.
-template <int BS, int OP>
-__global__ __launch_bounds__(256) void k_prefill_rows(const SpmvArgs a) {
-  const int64_t R = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (R >= a.nbr) return;
-  const int64_t rs = a.rowptr[R], re = a.rowptr[R + 1];
-  const bool empty = rs == re;
-  const bool split = !empty && (rs / a.chunk != (re - 1) / a.chunk);
-  if (!(empty || split)) return;
-  if constexpr (OP == OP_COUNT) {
-    if (!a.accumulate)
-      for (int r = 0; r < BS && R * BS + r < a.nrows; ++r) ((int32_t*)a.out)[R * BS + r] = 0;
-  } else if constexpr (OP == OP_BFS) {
-    ((word_t<BS>*)a.out)[R] = (word_t<BS>)0;
-  }
-}
-
-// Segmented lane-per-tile kernel for bs 8 and 16 (COUNT).  Every wave step
-// takes the next 64 tiles of its chunk whatever rows they belong to: lane l
-// finds the block row of its tile by a binary search (ds_bpermute) over a
-// window of 64 rowptr entries, per-row popcounts are formed SWAR-style on
-// the 64-bit tile words, and a segmented inclusive scan over the lanes adds
-// the tiles of each row; the last lane of a row segment writes the row (or
-// carries it into the next step when the row continues).  Loads of step
-// k+1 are issued before step k is reduced.
+// Streamed lane-per-tile COUNT kernel for bs 8 and 16.  The chunk's tiles
+// are read as one contiguous stream whatever rows they belong to: lane l
+// owns TPL tiles of every wave step, in vector groups of 2 consecutive
+// tiles (a 16-byte load for bs=8, four for bs=16, and an 8-byte load of
+// their column indices), and the loads of step k+1 are issued before step k
+// is reduced, so a wave keeps a whole step in flight across row boundaries
+// and the first step is issued before the row walk starts.  Per-row popcounts are formed
+// SWAR-style in 16-bit fields; the block rows are walked with the 64-row
+// rowptr window of k_spmv, and a row's fields are reduced across the wave
+// only when the row ends.
 template <int BS>
-__device__ __forceinline__ void unpack_counts(const uint64_t* v, uint32_t* cnt) {
-  if constexpr (BS == 8) {
+struct StreamSlots {
+  static constexpr int TPL = (BS == 8) ? 4 : 2;  // tiles per lane per step
+  static constexpr int GT = 2;                   // consecutive tiles per vector group
+  static constexpr int NG = TPL / GT;            // vector groups per lane per step
+  static constexpr int NWT = BS * BS / 64;       // 64-bit words per tile
+  uint64_t w[TPL * NWT];
+  int32_t col[TPL];
+  // Tile index of slot u in the step starting at tile ts.  Group q of the
+  // wave covers 64*GT consecutive tiles, GT per lane, so each vector load
+  // instruction reads one contiguous block (1 KB of bs=8 tiles).
+  static __device__ __forceinline__ int64_t tile(int64_t ts, int lane, int u) {
+    return ts + (int64_t)(u / GT) * (kWave * GT) + (int64_t)lane * GT + (u % GT);
+  }
+};
+
+// Loads the step starting at tile ts (a multiple of TPL).  A vector group
+// with no tile in [lo, hi) is not read; a group that runs past the last
+// tile is read tile by tile.  Slots that are not read get column 0 and zero
+// words.
+template <int BS>
+__device__ __forceinline__ void stream_load(const SpmvArgs& a, int64_t ts, int lane, int64_t lo, int64_t hi,
+                                            StreamSlots<BS>& s) {
+  typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+  typedef int i32x2 __attribute__((ext_vector_type(2)));
+  using S = StreamSlots<BS>;
+  constexpr int GT = S::GT, NWT = S::NWT;
 #pragma unroll
-    for (int f = 0; f < 4; ++f) {
-      cnt[2 * f] = (uint32_t)(v[0] >> (16 * f)) & 0xFFFFu;
-      cnt[2 * f + 1] = (uint32_t)(v[1] >> (16 * f)) & 0xFFFFu;
+  for (int q = 0; q < S::NG; ++q) {
+    const int64_t g0 = S::tile(ts, lane, q * GT);
+    const bool touch = (g0 + GT > lo) && (g0 < hi);
+    if (touch && g0 + GT <= a.T) {
+      const u32x4* tp = (const u32x4*)((const uint64_t*)a.tiles + g0 * NWT);
+#pragma unroll
+      for (int k = 0; k < GT * NWT / 2; ++k) {
+        const u32x4 v = __builtin_nontemporal_load(tp + k);
+        s.w[q * GT * NWT + 2 * k] = ((uint64_t)v.y << 32) | v.x;
+        s.w[q * GT * NWT + 2 * k + 1] = ((uint64_t)v.w << 32) | v.z;
+      }
+      const i32x2 cv = __builtin_nontemporal_load((const i32x2*)(a.colind + g0));
+      s.col[q * GT] = cv.x;
+      s.col[q * GT + 1] = cv.y;
+    } else {
+#pragma unroll
+      for (int e = 0; e < GT; ++e) {
+        const int64_t t = g0 + e;
+        const bool ok = touch && t < a.T;
+#pragma unroll
+        for (int k = 0; k < NWT; ++k) s.w[(q * GT + e) * NWT + k] = ok ? ((const uint64_t*)a.tiles)[t * NWT + k] : 0ull;
+        s.col[q * GT + e] = ok ? a.colind[t] : 0;
+      }
     }
-  } else {
-#pragma unroll
-    for (int k = 0; k < 4; ++k)
-#pragma unroll
-      for (int f = 0; f < 4; ++f) cnt[4 * k + f] = (uint32_t)(v[k] >> (16 * f)) & 0xFFFFu;
   }
 }
 
 template <int BS>
-__global__ __launch_bounds__(256) void k_count_seg(const SpmvArgs a) {
+__global__ __launch_bounds__(256) void k_count_stream(const SpmvArgs a) {
   using W = word_t<BS>;
-  constexpr int NW = BS * BS / 64;      // 64-bit words per tile
-  constexpr int NA = (BS == 8) ? 2 : 4;  // packed 16-bit-field sums
+  using S = StreamSlots<BS>;
+  constexpr int TPL = S::TPL, NWT = S::NWT;
+  constexpr int NA = (BS == 8) ? 2 : 4;  // packed 16-bit-field accumulators
+  constexpr int LPR = kWave / BS;        // lanes per row after the reduce-scatter
+  constexpr int64_t STEP = (int64_t)kWave * TPL;
   const int lane = lane_id();
-  const int c = (int)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave);
-  if (c >= a.nchunks) return;
+  const int c = wave_index();
+  if (c >= a.nchunks) return;  // wave-uniform
   const int64_t cb = (int64_t)c * a.chunk;
   const int64_t ce = min(cb + (int64_t)a.chunk, a.T);
-  if (cb >= ce) return;
-  const int32_t* __restrict__ colind = a.colind;
+  const bool last = (c == a.nchunks - 1);
   const W* __restrict__ xb = (const W*)a.xbits;
   int32_t* __restrict__ y = (int32_t*)a.out;
+  const int64_t base = cb - cb % TPL;
 
-  int32_t R0 = a.plan[c];
-  int64_t rsw, wend;
+  S nxt;
+  stream_load<BS>(a, base, lane, cb, ce, nxt);
+
+  // 64-row rowptr window: lane l holds row W0 + l
+  int32_t W0 = a.plan[c];
+  int32_t rsl = 0, rel = 0;
+  uint64_t work = 0;
+  bool more = false;
   auto load_window = [&]() {
-    rsw = (R0 + lane <= a.nbr) ? (int64_t)a.rowptr[R0 + lane] : INT64_MAX;
-    wend = (R0 + kWave <= a.nbr) ? (int64_t)a.rowptr[R0 + kWave] : a.T;
+    const int32_t Rl = W0 + lane;
+    rsl = 0;
+    rel = 0;
+    if (Rl < a.nbr) {
+      rsl = a.rowptr[Rl];
+      rel = a.rowptr[Rl + 1];
+    }
+    const bool belong = (Rl < a.nbr) && (last || (int64_t)rsl < ce);
+    more = (__ballot(belong) == ~0ull);
+    if (belong && rel == rsl) write_empty_row<BS, OP_COUNT>(a, Rl);
+    work = __ballot(belong && rel > rsl);
   };
+  int32_t R = 0;
+  int64_t rs = 0, re = 0;
+  auto next_row = [&]() -> bool {
+    while (!work) {
+      if (!more) return false;
+      W0 += kWave;
+      load_window();
+    }
+    const int j = __builtin_ctzll(work);
+    work &= work - 1;
+    R = W0 + j;
+    rs = __builtin_amdgcn_readlane(rsl, j);
+    re = __builtin_amdgcn_readlane(rel, j);
+    return true;
+  };
+
+  uint32_t v[BS];
+  uint64_t acc[NA];
+#pragma unroll
+  for (int k = 0; k < BS; ++k) v[k] = 0;
+#pragma unroll
+  for (int k = 0; k < NA; ++k) acc[k] = 0;
+  auto flush = [&]() {
+    // field f of acc[k]: bs 8 -> row 2f + k; bs 16 -> row 4k + f
+#pragma unroll
+    for (int k = 0; k < NA; ++k) {
+#pragma unroll
+      for (int f = 0; f < 4; ++f) {
+        const uint32_t val = (uint32_t)(acc[k] >> (16 * f)) & 0xFFFFu;
+        if constexpr (BS == 8) v[2 * f + k] += val;
+        else v[4 * k + f] += val;
+      }
+      acc[k] = 0;
+    }
+  };
+  auto finish_row = [&]() {
+    flush();
+    const uint32_t tot = reduce_scatter_rows<BS>(v);
+    const int64_t i = (int64_t)R * BS + lane / LPR;
+    if ((lane % LPR) == 0 && i < a.nrows) {
+      if (rs >= cb && re <= ce) {
+        if (a.accumulate) y[i] += (int32_t)tot;
+        else y[i] = (int32_t)tot;
+      } else if (tot) {
+        atomicAdd(&y[i], (int32_t)tot);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < BS; ++k) v[k] = 0;
+  };
+
   load_window();
-
-  auto flush_row = [&](int64_t R, const uint32_t* cnt, bool full) {
-    for (int r = 0; r < BS; ++r) {
-      const int64_t i = R * BS + r;
-      if (i >= a.nrows) break;
-      if (full) {
-        if (a.accumulate) y[i] += (int32_t)cnt[r];
-        else y[i] = (int32_t)cnt[r];
-      } else if (cnt[r]) {
-        atomicAdd(&y[i], (int32_t)cnt[r]);
-      }
-    }
-  };
-
-  uint32_t carry[BS];
+  bool open = next_row();  // a row is being accumulated
+  int since = 0;           // steps added since the last flush (fields stay < 2^16)
+  for (int64_t ts = base; open && ts < ce; ts += STEP) {
+    const S cur = nxt;
+    if (ts + STEP < ce) stream_load<BS>(a, ts + STEP, lane, cb, ce, nxt);
+    // per-slot per-row popcounts: bytes (bs 8) or 16-bit fields (bs 16)
+    uint64_t m[TPL][NWT];
 #pragma unroll
-  for (int r = 0; r < BS; ++r) carry[r] = 0;
-  int64_t carry_row = -1;
-
-  int32_t ncol = 0;
-  uint64_t nw[NW];
-  {
-    const int64_t t = cb + lane;
-    if (t < ce) {
-      ncol = colind[t];
-      load_tile<BS>(a.tiles, t, nw);
-    }
-  }
-  for (int64_t t0 = cb; t0 < ce; t0 += kWave) {
-    const int64_t t = t0 + lane;
-    const bool valid = t < ce;
-    const int32_t col = ncol;
-    uint64_t w[NW];
+    for (int u = 0; u < TPL; ++u) popc_fields<BS>(&cur.w[u * NWT], (uint64_t)xb[cur.col[u]], m[u]);
+    const int64_t tend = min(ts + STEP, ce);
+    while (true) {
+      const int64_t lo = rs > cb ? rs : cb, hi = re < tend ? re : tend;
+      uint64_t sum[NWT];
 #pragma unroll
-    for (int k = 0; k < NW; ++k) w[k] = nw[k];
-    {
-      const int64_t tn = t + kWave;  // prefetch the next step
-      if (tn < ce) {
-        ncol = colind[tn];
-        load_tile<BS>(a.tiles, tn, nw);
+      for (int k = 0; k < NWT; ++k) sum[k] = 0;
+#pragma unroll
+      for (int u = 0; u < TPL; ++u) {
+        const int64_t tu = S::tile(ts, lane, u);
+        const bool in = (tu >= lo) && (tu < hi);
+#pragma unroll
+        for (int k = 0; k < NWT; ++k) sum[k] += in ? m[u][k] : 0ull;
       }
-    }
-    uint64_t m[NA];
-    {
-      const uint64_t x = valid ? (uint64_t)xb[col] : 0ull;
-      uint64_t pf[NW];
-      popc_fields<BS>(w, x, pf);
+      // bs 8: at most TPL*8 = 32 per byte, so the byte sums do not carry
       if constexpr (BS == 8) {
-        m[0] = pf[0] & 0x00FF00FF00FF00FFull;
-        m[1] = (pf[0] >> 8) & 0x00FF00FF00FF00FFull;
+        acc[0] += sum[0] & 0x00FF00FF00FF00FFull;
+        acc[1] += (sum[0] >> 8) & 0x00FF00FF00FF00FFull;
       } else {
 #pragma unroll
-        for (int k = 0; k < 4; ++k) m[k] = pf[k];
+        for (int k = 0; k < 4; ++k) acc[k] += sum[k];
       }
+      if (re > tend) break;  // the row goes on past this step
+      finish_row();
+      since = 0;
+      open = next_row();
+      if (!open || rs >= tend) break;
     }
-    bool pending = valid;
-    while (__ballot(pending)) {
-      const bool inwin = pending && t < wend;
-      const uint64_t inmask = __ballot(inwin);
-      if (inmask) {
-        // last window slot whose row start is <= t
-        int pos = 0;
-#pragma unroll
-        for (int k = 32; k > 0; k >>= 1) {
-          const int cand = pos + k;
-          const int64_t v = __shfl(rsw, cand & (kWave - 1));
-          if (cand < kWave && v <= t) pos = cand;
-        }
-        const int64_t key = inwin ? (int64_t)R0 + pos : -2 - lane;  // never equals a row or -1
-        uint64_t v[NA];
-#pragma unroll
-        for (int k = 0; k < NA; ++k) v[k] = inwin ? m[k] : 0ull;
-        // segmented inclusive scan (rows are contiguous in lane order)
-#pragma unroll
-        for (int off = 1; off < kWave; off <<= 1) {
-          const int64_t kl = __shfl_up(key, off);
-#pragma unroll
-          for (int k = 0; k < NA; ++k) {
-            const uint64_t o = (uint64_t)__shfl_up((long long)v[k], off);
-            if (lane >= off && kl == key) v[k] += o;
-          }
-        }
-        const int64_t knext = __shfl_down(key, 1);
-        const bool seg_end = inwin && (lane == kWave - 1 || knext != key);
-        const int64_t rsR = __shfl(rsw, pos);
-        const int64_t reR0 = __shfl(rsw, (pos + 1) & (kWave - 1));
-        const int64_t reR = (pos == kWave - 1) ? wend : reR0;
-        const bool cont = seg_end && (reR > t + 1);  // row goes on past this tile
-        const int lastl = 63 - __builtin_clzll(inmask);
-        uint32_t cnt[BS];
-        unpack_counts<BS>(v, cnt);
-        if (key == carry_row) {
-#pragma unroll
-          for (int r = 0; r < BS; ++r) cnt[r] += carry[r];
-        }
-        if (seg_end && !cont) flush_row(key, cnt, rsR >= cb && reR <= ce);
-        const bool cont_any = __shfl((int)cont, lastl) != 0;
-        if (cont_any) {
-#pragma unroll
-          for (int r = 0; r < BS; ++r) carry[r] = (uint32_t)__shfl((int)cnt[r], lastl);
-          carry_row = __shfl(key, lastl);
-        } else {
-          carry_row = -1;
-        }
-        pending = pending && !inwin;
-      }
-      if (__ballot(pending)) {
-        R0 += kWave;
-        load_window();
-      }
+    if (++since >= 1024) {  // <= 32 per field per step
+      flush();
+      since = 0;
     }
   }
-  // a row that continues past the chunk: partial, combined atomically
-  if (carry_row >= 0 && lane == 0) flush_row(carry_row, carry, false);
+  if (open) finish_row();  // a row that continues past the chunk: atomic partial
 }
 
 // Initializes the outputs of split rows (rows that begin before the chunk
@@ -672,17 +759,20 @@ __global__ __launch_bounds__(256) void k_sum_fixup(const SpmvArgs a) {
   const int32_t R = a.plan[c + 1];
   const int64_t rs = a.rowptr[R], re = a.rowptr[R + 1];
   if (!(rs < nb) || rs < c * a.chunk) return;  // not split here, or not the owner
-  float s = a.partial[(c * 2 + 1) * BS + r], comp = 0.0f;
-  for (int64_t c2 = c + 1; c2 < a.nchunks && c2 * a.chunk < re; ++c2) kahan_add(s, comp, a.partial[(c2 * 2 + 0) * BS + r]);
+  double s = a.partial[(c * 2 + 1) * BS + r];
+  for (int64_t c2 = c + 1; c2 < a.nchunks && c2 * a.chunk < re; ++c2) s += a.partial[(c2 * 2 + 0) * BS + r];
   const int64_t i = (int64_t)R * BS + r;
-  if (i < a.nrows) ((float*)a.out)[i] = a.scale * (s + comp) + a.bias;
+  if (i < a.nrows) ((float*)a.out)[i] = (float)((double)a.scale * s + (double)a.bias);
 }
 
 template <int BS, int OP>
 static int spmv_launch_t(const SpmvArgs& a, hipStream_t st) {
-  const bool seg = (BS <= 16 && OP == OP_COUNT && a.variant == 2);
-  const bool lpt = (BS <= 16 && (OP == OP_COUNT || OP == OP_BFS) && a.variant >= 1 && !seg);
-  const bool need_prefill = !seg && (OP == OP_BFS || OP == OP_MINPLUS || OP == OP_MIN || (OP == OP_COUNT && !a.accumulate));
+  // variant 2 (streamed) reads 16- or 8-byte vectors of tiles and column
+  // indices: it needs 16-byte-aligned arrays, otherwise variant 0 runs.
+  const bool aligned = ((((uintptr_t)a.tiles) | ((uintptr_t)a.colind)) & 15) == 0;
+  const bool stream = (BS <= 16 && OP == OP_COUNT && a.variant == 2 && aligned);
+  const bool lpt = (BS <= 16 && (OP == OP_COUNT || OP == OP_BFS) && a.variant == 1);
+  const bool need_prefill = OP == OP_BFS || OP == OP_MINPLUS || OP == OP_MIN || (OP == OP_COUNT && !a.accumulate);
   if (need_prefill && a.nchunks > 1) {
     const int64_t n = (int64_t)(a.nchunks - 1) * BS;
     k_prefill_split<BS, OP><<<(unsigned)((n + 255) / 256), 256, 0, st>>>(a);
@@ -690,10 +780,8 @@ static int spmv_launch_t(const SpmvArgs& a, hipStream_t st) {
   }
   const int64_t blocks = ((int64_t)a.nchunks + 3) / 4;
   if constexpr (BS <= 16 && OP == OP_COUNT) {
-    if (seg) {
-      k_prefill_rows<BS, OP><<<(unsigned)((a.nbr + 255) / 256), 256, 0, st>>>(a);
-      GBO_LAUNCH_CHECK();
-      k_count_seg<BS><<<(unsigned)blocks, 256, 0, st>>>(a);
+    if (stream) {
+      k_count_stream<BS><<<(unsigned)blocks, 256, 0, st>>>(a);
     } else if (lpt) {
       k_spmv_lpt<BS, OP><<<(unsigned)blocks, 256, 0, st>>>(a);
     } else {
@@ -820,9 +908,9 @@ int gbo_bsrbmv_sum_f32(const gbo_bsr* A, const float* d_x, float* d_y, gbo_strea
   int rc = validate_bsr(A, 0, "gbo_bsrbmv_sum_f32");
   if (rc) return rc;
   if (A->nrows > 0 && (!d_x || !d_y)) return set_error(GBO_EINVAL, "gbo_bsrbmv_sum_f32: null vector");
-  float* partial = nullptr;
+  double* partial = nullptr;
   rc = run_op(A, OP_SUM, stream, [&](SpmvArgs& a) -> int {
-    GBO_HIP_CHECK(hipMallocAsync((void**)&partial, sizeof(float) * 2 * (size_t)a.nchunks * A->bs, stream));
+    GBO_HIP_CHECK(hipMallocAsync((void**)&partial, sizeof(double) * 2 * (size_t)a.nchunks * A->bs, stream));
     a.xval = d_x;
     a.out = d_y;
     a.partial = partial;

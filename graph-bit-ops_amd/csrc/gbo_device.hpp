@@ -21,6 +21,14 @@ template <int BS> using word_t = typename WordOf<BS>::type;
 
 __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & (kWave - 1)); }
 
+// Index of the calling wavefront in the grid.  threadIdx.x / 64 is the same
+// in every lane, but the compiler cannot prove it; readfirstlane makes the
+// value (and what is derived from it) scalar, so per-chunk loop bounds live
+// in SGPRs and their branches are uniform.
+__device__ __forceinline__ int wave_index() {
+  return __builtin_amdgcn_readfirstlane((int)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave));
+}
+
 __device__ __forceinline__ int popcw(uint64_t x) { return __popcll(x); }
 __device__ __forceinline__ int popcw(uint32_t x) { return __popc(x); }
 __device__ __forceinline__ int popcw(uint16_t x) { return __popc((uint32_t)x); }

@@ -49,7 +49,7 @@ struct SpmvArgs {
   const int64_t* woff;  // MINPLUS per-tile weight offsets
   const void* xval;     // MINPLUS dist / SUM x / MIN x
   int64_t* count;       // BFS: number of newly set bits
-  float* partial;       // SUM: split-row partials [nchunks][2][bs]
+  double* partial;      // SUM: split-row partials [nchunks][2][bs]
   const uint64_t* active;  // MINPLUS: optional per-block-column activity bits (nullable)
   int accumulate;       // COUNT: add into y
   float scale, bias;    // SUM: y = scale * (A x) + bias  (scale 1, bias 0 for the plain op)

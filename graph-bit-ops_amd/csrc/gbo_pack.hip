@@ -254,10 +254,10 @@ __global__ __launch_bounds__(256) void k_plan(const int32_t* __restrict__ rowptr
 
 int default_chunk(int bs) {
   switch (bs) {
-    case 8: return 4096;
-    case 16: return 2048;
-    case 32: return 512;
-    default: return 256;
+    case 8: return 512;
+    case 16: return 256;
+    case 32: return 256;
+    default: return 128;
   }
 }
 

@@ -78,7 +78,7 @@ template <int BS>
 __global__ __launch_bounds__(256) void k_ppr_spmm(const PprArgs A) {
   using W = word_t<BS>;
   const int lane = lane_id();
-  const int c = (int)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave);
+  const int c = wave_index();
   if (c >= A.nchunks) return;
   const int64_t cb = (int64_t)c * A.chunk;
   const int64_t ce = min(cb + (int64_t)A.chunk, A.T);
@@ -100,7 +100,7 @@ __global__ __launch_bounds__(256) void k_ppr_spmm(const PprArgs A) {
       const int j = __builtin_ctzll(todo);
       todo &= todo - 1;
       const int64_t R = R0 + j;
-      const int64_t rs = __shfl(rsl, j), re = __shfl(rel, j);
+      const int64_t rs = __builtin_amdgcn_readlane(rsl, j), re = __builtin_amdgcn_readlane(rel, j);
       const int64_t s = rs > cb ? rs : cb;
       const int64_t te = re < ce ? re : ce;
       const bool full = (rs >= cb) && (re <= ce);

@@ -177,9 +177,11 @@ void gbo_bsr_free(gbo_bsr* A);
 int gbo_bsr_plan(gbo_bsr* A, int32_t chunk_tiles, gbo_stream_t stream);
 /* As gbo_bsr_plan, also selecting the bs 8/16 kernel variant of the COUNT
  * and BFS products: 0 = one lane group per tile row word (default),
- * 1 = one lane per tile, block rows one after another, 2 = one lane per
- * tile, segmented over block rows (COUNT only; BFS uses 1).  Results are
- * identical; only speed differs (the autotuner picks). */
+ * 1 = one lane per tile, block rows one after another, 2 = streamed: each
+ * lane reads 4 (bs 8) or 2 (bs 16) consecutive tiles per step with the next
+ * step's loads in flight across block-row boundaries (COUNT only; BFS runs
+ * variant 0; needs 16-byte-aligned tiles/colind, else variant 0 runs).
+ * Results are identical; only speed differs (the autotuner picks). */
 int gbo_bsr_plan_ex(gbo_bsr* A, int32_t chunk_tiles, int32_t variant, gbo_stream_t stream);
 void gbo_bsr_plan_free(gbo_bsr* A);
 /* Per-nonzero values in tile bit order (row-major, MSB-first inside a row

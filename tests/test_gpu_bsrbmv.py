@@ -12,6 +12,9 @@ from graphs import cases
 pytestmark = pytest.mark.gpu
 BS = (8, 16, 32, 64)
 CHUNKS = (0, 3)  # 0 = default chunk; 3 tiles = split rows everywhere
+# count also runs with a chunk that is not a multiple of the streamed
+# kernel's tiles per lane and spans several of its wave steps
+COUNT_CHUNKS = (0, 3, 1029)
 
 
 @pytest.fixture(scope="module")
@@ -26,7 +29,7 @@ def packed(gbo, g, bs, chunk, variant=0):
 
 
 @pytest.mark.parametrize("bs", BS)
-@pytest.mark.parametrize("chunk", CHUNKS)
+@pytest.mark.parametrize("chunk", COUNT_CHUNKS)
 @pytest.mark.parametrize("variant", (0, 1, 2))
 def test_count(gbo, po, graph_cases, bs, chunk, variant):
     if variant and bs > 16:
@@ -40,6 +43,31 @@ def test_count(gbo, po, graph_cases, bs, chunk, variant):
         # accumulate (+=) semantics of the reference kernels
         y0 = np.arange(g.nrows, dtype=np.int32)
         assert np.array_equal(gbo.bsrbmv_count(A, xb, y_init=y0), y + y0), name
+
+
+@pytest.mark.parametrize("bs", (8, 16, 32))
+@pytest.mark.parametrize("variant", (0, 1, 2))
+def test_count_long_rows(gbo, po, bs, variant):
+    """Block rows of >300k tiles, each row count > 2^16, in one chunk: the
+    per-lane 16-bit field accumulators of the bs 8/16 variants must be
+    flushed before they wrap."""
+    if variant and bs > 16:
+        pytest.skip("kernel variants exist for bs 8/16 only")
+    ncols, nrows = 3_000_000, 16
+    cols = [np.arange(r % 8, ncols, 7, dtype=np.int32) for r in range(nrows)]
+    rowptr = np.zeros(nrows + 1, np.int32)
+    rowptr[1:] = np.cumsum([len(c) for c in cols])
+    g = gbo.Csr(nrows, ncols, rowptr, np.concatenate(cols))
+    x = bernoulli_x(ncols, 0.5, 11)
+    xb = gbo.pack_vec(bs, x)
+    want = np.array([int(x[c].sum()) for c in cols])
+    assert want.min() > 1 << 16
+    for chunk in (0, 1 << 30):
+        A = gbo.BsrMatrix.from_csr(g, bs, 0, plan=False)
+        A.plan(chunk, variant)
+        y = gbo.bsrbmv_count(A, xb)
+        assert np.array_equal(y, want), chunk
+        A.free()
 
 
 def test_count_dense_bruteforce(gbo):
