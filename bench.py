@@ -225,6 +225,41 @@ def cpu_baseline(gbo, A, xbits, n, target_s):
     }
 
 
+def bench_tc(gbo, timer, reps=3):
+    """C3: triangle counting by bsrbmm (L row tiles x L^T column tiles,
+    masked by L, popcount, uint64 total) on the 2048x2048 king grid (closed
+    form 4(m-1)^2) and R-MAT s20 (seed 2).  The kernel is timed with HIP
+    events on prepacked operands; strict tril/transposition run on the host."""
+    out = {}
+    for name, g in (("king2048", gbo.gen_king(2048)), ("rmat20", gbo.gen_rmat(20, 16, seed=2))):
+        L = g.tril(strict=True)
+        LT = L.transpose()
+        rec = {"n": g.nrows, "nnz": g.nnz, "L_nnz": L.nnz}
+        if name == "king2048":
+            rec["closed_form"] = 4 * 2047 ** 2
+        per = {}
+        for bs in (8, 16, 32, 64):
+            Lr = gbo.BsrMatrix.from_csr(L, bs, 0)
+            Lc = gbo.BsrMatrix.from_csr(LT, bs, 1)
+            cnt = gbo.bsrbmm_tc(Lr, Lc)  # warm-up
+            timer.start()
+            for _ in range(reps):
+                cnt = gbo.bsrbmm_tc(Lr, Lc)
+            ms = timer.stop_ms() / reps
+            per[bs] = {"triangles": cnt, "ms": round(ms, 3), "L_tiles": Lr.nblocks,
+                       "GTEPS_L": round(L.nnz / (ms * 1e-3) / 1e9, 3)}
+            Lr.free()
+            Lc.free()
+        rec["per_block_size"] = per
+        counts = {v["triangles"] for v in per.values()}
+        rec["consistent_across_bs"] = len(counts) == 1
+        if "closed_form" in rec:
+            rec["matches_closed_form"] = counts == {rec["closed_form"]}
+        out[name] = rec
+        log(f"[bench] tc {name}: {rec}")
+    return out
+
+
 def keep_ppr(gbo, d_rp, d_ci, n):
     """The PPR SpMM runs at bs 8 (one 256-B row gather per set bit)."""
     return gbo.BsrMatrix.from_device_csr(d_rp, d_ci, n, n, 8, 0)
@@ -271,6 +306,7 @@ def main():
 
     per_bs = {}
     keep = {}
+    w_sssp = None
     for bs in [int(b) for b in args.bs.split(",")]:
         t = time.time()
         A = gbo.BsrMatrix.from_device_csr(d_rp, d_ci, n, n, bs, 0)
@@ -311,9 +347,25 @@ def main():
             t = time.time()
             p, it, delta = gbo.pagerank(A, outdeg, 0.85, 1e-7, 1000)
             pr_s = time.time() - t
+            # per iteration: one sum sweep (tiles, colind, rowptr, read p, write p)
+            pr_bytes = sweep_bytes(A, n) - 4 * n + 8 * n
             rec["pagerank"] = {"iters": it, "ms": round(pr_s * 1e3, 1), "ms_per_iter": round(pr_s * 1e3 / it, 3),
-                               "last_l1_delta": delta}
-            log(f"[bench] bs={bs} bfs {rec['bfs']} pr {rec['pagerank']}")
+                               "last_l1_delta": delta, "bytes_per_iter": pr_bytes,
+                               "frac_hbm_per_iter": round(pr_bytes / (pr_s / it) / 1e9 / HBM_PEAK_GBS, 4)}
+            if w_sssp is None:
+                w_sssp = gbo.edge_weights(g, 1, 64, 1)
+            tv = gbo.TileValues(A, g, w_sssp)
+            t = time.time()
+            dist, sit = gbo.sssp(A, src, tv)
+            sssp_s = time.time() - t
+            tv.free()
+            rec["sssp"] = {"source": src, "weights": "uint32 uniform [1,64], seed 1", "iters": sit,
+                           "reached": int((dist != 0xFFFFFFFF).sum()), "ms": round(sssp_s * 1e3, 1)}
+            t = time.time()
+            lab, cit = gbo.cc(A)
+            cc_s = time.time() - t
+            rec["cc"] = {"iters": cit, "components": int(len(np.unique(lab))), "ms": round(cc_s * 1e3, 1)}
+            log(f"[bench] bs={bs} bfs {rec['bfs']} pr {rec['pagerank']} sssp {rec['sssp']} cc {rec['cc']}")
         per_bs[bs] = rec
         keep[bs] = (A, dx)
         if len(keep) > 1:  # keep only the best so far resident
@@ -338,6 +390,13 @@ def main():
             except Exception as e:  # a failing side line must not hide the headline
                 extra[name] = {"error": str(e)}
                 log(f"[bench] {name} failed: {e}")
+    tc = None
+    if not args.no_algos:
+        try:
+            tc = bench_tc(gbo, timer)
+        except Exception as e:  # a failing side line must not hide the headline
+            tc = {"error": str(e)}
+            log(f"[bench] tc failed: {e}")
     cpu = None
     if not args.no_cpu:
         xb = gbo.pack_vec(bs, x)
@@ -354,6 +413,7 @@ def main():
                      "frac": best["frac_hbm"], "traffic": None, "copy_ceiling_GBs": copy_gbs},
         "cpu_baseline": cpu,
         "per_block_size": per_bs,
+        "triangle_counting": tc,
         "batched": extra,
     }
     print(json.dumps(out), flush=True)

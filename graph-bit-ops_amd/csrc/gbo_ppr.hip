@@ -157,8 +157,9 @@ __global__ __launch_bounds__(256) void k_ppr_fixup(const PprArgs A) {
 }
 
 // Deterministic column-wise reduction over a [n][ld] column block of
-// `cols` <= 64 columns; mode 0: sum |Y - P|, mode 1: sum L * (Y - P).  Each
-// block writes 64 partials; k_colred_final adds them in block order.
+// `cols` <= 64 columns; mode 0: sum |Y - P|, mode 1: sum L * (Y - P),
+// mode 2: sum |Y|.  Each block writes 64 partials; k_colred_final adds
+// them in block order.
 __global__ __launch_bounds__(256) void k_colred(const float* __restrict__ Y, const float* __restrict__ P,
                                                 const float* __restrict__ L, int32_t n, int32_t ld, int32_t cols,
                                                 int mode, double* __restrict__ part) {
@@ -168,7 +169,8 @@ __global__ __launch_bounds__(256) void k_colred(const float* __restrict__ Y, con
     for (int64_t i = (int64_t)blockIdx.x * 4 + rsub; i < n; i += (int64_t)gridDim.x * 4) {
       const int64_t o = i * ld + col;
       if (mode == 0) acc += fabs((double)Y[o] - (double)P[o]);
-      else acc += (double)L[o] * ((double)Y[o] - (double)P[o]);
+      else if (mode == 1) acc += (double)L[o] * ((double)Y[o] - (double)P[o]);
+      else acc += fabs((double)Y[o]);
     }
   }
   __shared__ double sh[256];
@@ -291,7 +293,7 @@ static int colreduce(const float* Y, const float* P, const float* L, int32_t n, 
   constexpr int nb = 256;
   for (int32_t k0 = 0; k0 < K; k0 += 64) {
     const int32_t cols = std::min(64, K - k0);
-    k_colred<<<nb, 256, 0, st>>>(Y + k0, P + k0, L ? L + k0 : nullptr, n, K, cols, mode, d_part);
+    k_colred<<<nb, 256, 0, st>>>(Y + k0, P ? P + k0 : nullptr, L ? L + k0 : nullptr, n, K, cols, mode, d_part);
     GBO_LAUNCH_CHECK();
     k_colred_final<<<1, 64, 0, st>>>(d_part, nb, d_out);
     GBO_LAUNCH_CHECK();
@@ -299,6 +301,13 @@ static int colreduce(const float* Y, const float* P, const float* L, int32_t n, 
   }
   GBO_HIP_CHECK(hipStreamSynchronize(st));
   return GBO_OK;
+}
+
+// max over columns of delta[k] / norm[k] (delta[k] itself where norm[k] is 0)
+static double max_relative(const std::vector<double>& delta, const std::vector<double>& norm) {
+  double m = 0.0;
+  for (size_t k = 0; k < delta.size(); ++k) m = std::max(m, norm[k] > 0.0 ? delta[k] / norm[k] : delta[k]);
+  return m;
 }
 
 }  // namespace gbo
@@ -330,7 +339,9 @@ int gbo_ppr_forward(const gbo_bsr* AT, const int32_t* d_outdeg, int32_t K, float
   rc = make_ppr_plan(AT, &P, st);
   // p_0 = s
   if (rc == GBO_OK) GBO_HIP_CHECK(hipMemcpyAsync(d_p, d_s, sizeof(float) * nk, hipMemcpyDeviceToDevice, st));
-  std::vector<double> delta(K);
+  std::vector<double> delta(K), snorm(K);
+  // the change is measured relative to ||s_k||_1 (1 for a distribution)
+  if (rc == GBO_OK) rc = colreduce(d_s, nullptr, nullptr, n, K, 2, dpart, dpart + 256 * 64, snorm.data(), st);
   int32_t it = 0;
   double dmax = 0.0;
   float* cur = d_p;
@@ -342,7 +353,7 @@ int gbo_ppr_forward(const gbo_bsr* AT, const int32_t* d_outdeg, int32_t K, float
     if (rc) break;
     ++it;
     std::swap(cur, nxt);
-    dmax = *std::max_element(delta.begin(), delta.end());
+    dmax = max_relative(delta, snorm);
     if (dmax < tol) break;
   }
   if (rc == GBO_OK && cur != d_p) GBO_HIP_CHECK(hipMemcpyAsync(d_p, cur, sizeof(float) * nk, hipMemcpyDeviceToDevice, st));
@@ -385,7 +396,9 @@ int gbo_ppr_adjoint(const gbo_bsr* A, const gbo_bsr* AT, const int32_t* d_outdeg
   rc = make_ppr_plan(A, &PA, st);
   if (rc == GBO_OK) rc = make_ppr_plan(AT, &PT, st);
   if (rc == GBO_OK) GBO_HIP_CHECK(hipMemcpyAsync(lam, d_w, sizeof(float) * nk, hipMemcpyDeviceToDevice, st));
-  std::vector<double> delta(K), ga(K);
+  std::vector<double> delta(K), ga(K), wnorm(K);
+  // the change is measured relative to ||w_k||_1, the scale of lambda_k
+  if (rc == GBO_OK) rc = colreduce(d_w, nullptr, nullptr, n, K, 2, dpart, dpart + 256 * 64, wnorm.data(), st);
   int32_t it = 0;
   float* cur = lam;
   float* nxt = q;
@@ -398,7 +411,7 @@ int gbo_ppr_adjoint(const gbo_bsr* A, const gbo_bsr* AT, const int32_t* d_outdeg
     if (rc) break;
     ++it;
     std::swap(cur, nxt);
-    dmax = *std::max_element(delta.begin(), delta.end());
+    dmax = max_relative(delta, wnorm);
     if (dmax < tol) break;
   }
   if (rc == GBO_OK) {

@@ -473,6 +473,10 @@ class TileValues:
         synchronize()
         self.count = int(self.woff.to_numpy(np.int64)[A.nblocks]) if A.nblocks else 0
 
+    def free(self):
+        self.tval.free()
+        self.woff.free()
+
 
 def bsrbmv_minplus(A: BsrMatrix, dist, weights: TileValues | None = None) -> np.ndarray:
     dd = _dev(dist, np.uint32)
@@ -577,7 +581,7 @@ def bfs_multisource(AT: BsrMatrix, sources, out: DeviceBuffer | None = None):
 def ppr_forward(AT: BsrMatrix, outdeg, S, alpha: float = 0.85, tol: float = 1e-7, max_iter: int = 1000,
                 strict: bool = False):
     """P[n][K] = personalized PageRank of the K columns of S[n][K] (float32).
-    Returns (P, iterations, last max-column L1 change)."""
+    Returns (P, iterations, last max-column L1 change relative to ||s_k||_1)."""
     S = np.ascontiguousarray(S, np.float32)
     n, K = S.shape
     do = DeviceBuffer.from_numpy(np.asarray(outdeg, np.int32))
@@ -593,7 +597,8 @@ def ppr_forward(AT: BsrMatrix, outdeg, S, alpha: float = 0.85, tol: float = 1e-7
 def ppr_adjoint(A: BsrMatrix, AT: BsrMatrix, outdeg, S, P, W, alpha: float = 0.85, tol: float = 1e-7,
                 max_iter: int = 1000, strict: bool = False):
     """Gradients of L = sum(W * P) with respect to S and alpha:
-    (grad_S[n][K] float32, grad_alpha float, iterations)."""
+    (grad_S[n][K] float32, grad_alpha float, iterations).  The adjoint solve
+    stops when the max-column L1 change relative to ||W_k||_1 is below tol."""
     S = np.ascontiguousarray(S, np.float32)
     n, K = S.shape
     do = DeviceBuffer.from_numpy(np.asarray(outdeg, np.int32))

@@ -329,15 +329,19 @@ void gbo_batch_free(gbo_batch* b);
  * ([n][K] float):  p_k = alpha * P^T p_k + (1 - alpha) * s_k,  P = D^-1 A,
  * rows of vertices without out-edges contribute nothing (as pr.cu).  The
  * pull product reads AT (A^T; A itself for a symmetric graph).  Iterates
- * from p = s until max_k ||p_k' - p_k||_1 < tol (GBO_ENOCONV after
- * max_iter).  No reference counterpart: generalises algorithm/pr.cu:449-516. */
+ * from p = s until max_k ||p_k' - p_k||_1 / ||s_k||_1 < tol (GBO_ENOCONV
+ * after max_iter; the relative change is reported in *h_delta).  No
+ * reference counterpart: generalises algorithm/pr.cu:449-516. */
 int gbo_ppr_forward(const gbo_bsr* AT, const int32_t* d_outdeg, int32_t K, float alpha, const float* d_s,
                     float* d_p, double tol, int32_t max_iter, int32_t* h_iters, double* h_delta,
                     gbo_stream_t stream);
 /* Adjoint of gbo_ppr_forward for L = sum_k w_k . p_k: solves
  * lambda_k = alpha * P lambda_k + w_k (pull over A), then
  * d_grad_s = (1 - alpha) * lambda and *h_grad_alpha =
- * sum_k lambda_k . (P^T p_k - s_k).  Uses only the converged p. */
+ * sum_k lambda_k . (P^T p_k - s_k).  Uses only the converged p.  Iterates
+ * from lambda = w until max_k ||lambda_k' - lambda_k||_1 / ||w_k||_1 < tol
+ * (the same contraction as the forward solve, so the same tol needs about
+ * the same number of iterations). */
 int gbo_ppr_adjoint(const gbo_bsr* A, const gbo_bsr* AT, const int32_t* d_outdeg, int32_t K, float alpha,
                     const float* d_s, const float* d_p, const float* d_w, float* d_grad_s, double* h_grad_alpha,
                     double tol, int32_t max_iter, int32_t* h_iters, gbo_stream_t stream);

@@ -116,6 +116,19 @@ def test_tc_vs_oracle(gbo, po, graph_cases, bs):
         assert gbo.tc(g, bs) == want, name
 
 
+def test_tc_full_size_workloads(gbo, po):
+    """The full-size triangle-counting workloads: the 2048x2048 king grid
+    (closed form 4*2047^2 = 16,760,836) and R-MAT s20 (seed 2) against the
+    oracle's merge count, at the smallest and largest block sizes."""
+    king = gbo.gen_king(2048)
+    for bs in (8, 64):
+        assert gbo.tc(king, bs) == 4 * 2047 ** 2
+    g = gbo.gen_rmat(20, 16, seed=2)
+    want = po.tc(g.nrows, g.rowptr, g.colind)
+    for bs in (8, 64):
+        assert gbo.tc(g, bs) == want
+
+
 @pytest.mark.parametrize("bs", BS)
 def test_bsrbmm_operands(gbo, po, bs):
     """gbo_bsrbmm_tc / gbo_bsrbmm_sum on explicit (L row, L^T col) operands,
