@@ -35,6 +35,22 @@ def gbo_default_chunk(bs):
     return {8: 512, 16: 256, 32: 256, 64: 128}[bs]
 
 
+PROFILE_MD = os.path.join(ROOT, "profiles", "bsrbmv_count_rmat_s22.md")
+
+
+def measured_traffic(bs, variant, scale):
+    """HBM bytes per count-sweep launch (calibrated FETCH_SIZE + WRITE_SIZE)
+    for this block size and kernel variant, from the committed rocprofv3
+    summary of the same workload (R-MAT s22 only), or None."""
+    if scale != 22 or not os.path.exists(PROFILE_MD):
+        return None
+    for line in open(PROFILE_MD):
+        if line.startswith(f"traffic: bs={bs} variant={variant} "):
+            f = dict(kv.split("=") for kv in line.split()[1:])
+            return int(f["fetch_bytes"]) + int(f["write_bytes"])
+    return None
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -76,6 +92,28 @@ def time_count(gbo, A, dx, dy, steps, warmup, timer):
     for _ in range(steps):
         gbo._check(L.gbo_bsrbmv_count(A.ref, dx.ptr, dy.ptr, 0, None))
     return timer.stop_ms() / steps
+
+
+def time_bfs_step(gbo, A, dx, steps, warmup, timer):
+    """One pull-BFS step over every tile: frontier = the count sweep's x bits
+    (half the vertices), nothing visited."""
+    L = gbo.lib()
+    nwb = (A.nblockcols * A.bs) // 8
+    vis = gbo.DeviceBuffer(nwb)
+    gbo._check(L.gbo_memset(vis.ptr, 0, nwb))
+    nxt = gbo.DeviceBuffer(max(A.nblockrows, 1) * A.bs // 8)
+    cnt = gbo.DeviceBuffer(8)
+    for _ in range(warmup):
+        gbo._check(L.gbo_bsrbmv_bfs_step(A.ref, dx.ptr, vis.ptr, nxt.ptr, cnt.ptr, None))
+    gbo.synchronize()
+    timer.start()
+    for _ in range(steps):
+        gbo._check(L.gbo_bsrbmv_bfs_step(A.ref, dx.ptr, vis.ptr, nxt.ptr, cnt.ptr, None))
+    ms = timer.stop_ms() / steps
+    vis.free()
+    nxt.free()
+    cnt.free()
+    return ms
 
 
 def copy_ceiling(gbo, timer, nbytes=1 << 30, reps=10):
@@ -337,16 +375,33 @@ def main():
             f"{rec['GTEPS']} GTEPS {rec['achieved_GBs']} GB/s")
         if not args.no_algos:
             src = int(np.argmax(outdeg > 0))
-            t = time.time()
-            lv, sizes = gbo.bfs(A, src)
-            bfs_s = time.time() - t
+            # every driver runs once untimed (device memory pools, plans), then
+            # the median of `reps` timed runs is reported
+            reps = 3
+
+            def timed(fn):
+                fn()
+                gbo.synchronize()
+                ts, r = [], None
+                for _ in range(reps):
+                    t = time.perf_counter()
+                    r = fn()
+                    gbo.synchronize()
+                    ts.append(time.perf_counter() - t)
+                return float(np.median(ts)), r
+
+            bfs_s, (lv, sizes) = timed(lambda: gbo.bfs(A, src))
             reached = lv >= 0
             edges = int(outdeg[reached].sum()) // 2
+            # one BFS-step sweep with half the vertices in the frontier and none
+            # visited: every tile is read (tiles, colind, rowptr, 3 bit vectors)
+            step_ms = time_bfs_step(gbo, A, dx, args.steps, args.warmup, timer)
+            step_bytes = sweep_bytes(A, n) - 4 * n + 3 * ((n + 7) // 8)
             rec["bfs"] = {"source": src, "levels": len(sizes), "reached": int(reached.sum()),
-                          "ms": round(bfs_s * 1e3, 2), "graph500_GTEPS": round(edges / bfs_s / 1e9, 3)}
-            t = time.time()
-            p, it, delta = gbo.pagerank(A, outdeg, 0.85, 1e-7, 1000)
-            pr_s = time.time() - t
+                          "ms": round(bfs_s * 1e3, 2), "graph500_GTEPS": round(edges / bfs_s / 1e9, 3),
+                          "full_step_ms": round(step_ms, 4), "full_step_bytes": step_bytes,
+                          "full_step_frac_hbm": round(step_bytes / (step_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+            pr_s, (p, it, delta) = timed(lambda: gbo.pagerank(A, outdeg, 0.85, 1e-7, 1000))
             # per iteration: one sum sweep (tiles, colind, rowptr, read p, write p)
             pr_bytes = sweep_bytes(A, n) - 4 * n + 8 * n
             rec["pagerank"] = {"iters": it, "ms": round(pr_s * 1e3, 1), "ms_per_iter": round(pr_s * 1e3 / it, 3),
@@ -355,15 +410,11 @@ def main():
             if w_sssp is None:
                 w_sssp = gbo.edge_weights(g, 1, 64, 1)
             tv = gbo.TileValues(A, g, w_sssp)
-            t = time.time()
-            dist, sit = gbo.sssp(A, src, tv)
-            sssp_s = time.time() - t
+            sssp_s, (dist, sit) = timed(lambda: gbo.sssp(A, src, tv))
             tv.free()
             rec["sssp"] = {"source": src, "weights": "uint32 uniform [1,64], seed 1", "iters": sit,
                            "reached": int((dist != 0xFFFFFFFF).sum()), "ms": round(sssp_s * 1e3, 1)}
-            t = time.time()
-            lab, cit = gbo.cc(A)
-            cc_s = time.time() - t
+            cc_s, (lab, cit) = timed(lambda: gbo.cc(A))
             rec["cc"] = {"iters": cit, "components": int(len(np.unique(lab))), "ms": round(cc_s * 1e3, 1)}
             log(f"[bench] bs={bs} bfs {rec['bfs']} pr {rec['pagerank']} sssp {rec['sssp']} cc {rec['cc']}")
         per_bs[bs] = rec
@@ -410,7 +461,8 @@ def main():
                                f"symmetrized, dedup, seed 1), x=Bernoulli(0.5)",
                    "nnz": nnz, "bs": bs, "parallelism": "none (1 GPU)"},
         "roofline": {"bound": "hbm", "achieved": best["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": best["frac_hbm"], "traffic": None, "copy_ceiling_GBs": copy_gbs},
+                     "frac": best["frac_hbm"], "traffic": measured_traffic(bs, best["variant"], args.scale),
+                     "algorithmic_bytes": best["bytes_per_sweep"], "copy_ceiling_GBs": copy_gbs},
         "cpu_baseline": cpu,
         "per_block_size": per_bs,
         "triangle_counting": tc,
