@@ -590,8 +590,22 @@ __device__ __forceinline__ void stream_load(const SpmvArgs& a, int64_t ts, int l
   }
 }
 
+// Rows W0 .. W0+63 of the rowptr window, lane l holding row W0 + l.
+__device__ __forceinline__ void window_load(const SpmvArgs& a, int32_t W0, int lane, int32_t& rsl, int32_t& rel) {
+  const int32_t Rl = W0 + lane;
+  rsl = 0;
+  rel = 0;
+  if (Rl < a.nbr) {
+    rsl = a.rowptr[Rl];
+    rel = a.rowptr[Rl + 1];
+  }
+}
+
+// One chunk of the streamed count: `nxt` holds the chunk's first step
+// (already issued) and rsl/rel the rowptr window starting at W0 = plan[c].
 template <int BS>
-__global__ __launch_bounds__(256) void k_count_stream(const SpmvArgs a) {
+__device__ __forceinline__ void stream_chunk(const SpmvArgs& a, int c, StreamSlots<BS>& nxt, int32_t W0,
+                                             int32_t rsl, int32_t rel) {
   using W = word_t<BS>;
   using S = StreamSlots<BS>;
   constexpr int TPL = S::TPL, NWT = S::NWT;
@@ -599,8 +613,6 @@ __global__ __launch_bounds__(256) void k_count_stream(const SpmvArgs a) {
   constexpr int LPR = kWave / BS;        // lanes per row after the reduce-scatter
   constexpr int64_t STEP = (int64_t)kWave * TPL;
   const int lane = lane_id();
-  const int c = wave_index();
-  if (c >= a.nchunks) return;  // wave-uniform
   const int64_t cb = (int64_t)c * a.chunk;
   const int64_t ce = min(cb + (int64_t)a.chunk, a.T);
   const bool last = (c == a.nchunks - 1);
@@ -608,22 +620,10 @@ __global__ __launch_bounds__(256) void k_count_stream(const SpmvArgs a) {
   int32_t* __restrict__ y = (int32_t*)a.out;
   const int64_t base = cb - cb % TPL;
 
-  S nxt;
-  stream_load<BS>(a, base, lane, cb, ce, nxt);
-
-  // 64-row rowptr window: lane l holds row W0 + l
-  int32_t W0 = a.plan[c];
-  int32_t rsl = 0, rel = 0;
   uint64_t work = 0;
   bool more = false;
-  auto load_window = [&]() {
+  auto apply_window = [&]() {
     const int32_t Rl = W0 + lane;
-    rsl = 0;
-    rel = 0;
-    if (Rl < a.nbr) {
-      rsl = a.rowptr[Rl];
-      rel = a.rowptr[Rl + 1];
-    }
     const bool belong = (Rl < a.nbr) && (last || (int64_t)rsl < ce);
     more = (__ballot(belong) == ~0ull);
     if (belong && rel == rsl) write_empty_row<BS, OP_COUNT>(a, Rl);
@@ -635,7 +635,8 @@ __global__ __launch_bounds__(256) void k_count_stream(const SpmvArgs a) {
     while (!work) {
       if (!more) return false;
       W0 += kWave;
-      load_window();
+      window_load(a, W0, lane, rsl, rel);
+      apply_window();
     }
     const int j = __builtin_ctzll(work);
     work &= work - 1;
@@ -680,7 +681,7 @@ __global__ __launch_bounds__(256) void k_count_stream(const SpmvArgs a) {
     for (int k = 0; k < BS; ++k) v[k] = 0;
   };
 
-  load_window();
+  apply_window();
   bool open = next_row();  // a row is being accumulated
   int since = 0;           // steps added since the last flush (fields stay < 2^16)
   for (int64_t ts = base; open && ts < ce; ts += STEP) {
@@ -720,6 +721,233 @@ __global__ __launch_bounds__(256) void k_count_stream(const SpmvArgs a) {
     if (++since >= 1024) {  // <= 32 per field per step
       flush();
       since = 0;
+    }
+  }
+  if (open) finish_row();  // a row that continues past the chunk: atomic partial
+}
+
+// First tile of chunk c's first step (a multiple of the tiles per lane).
+template <int BS>
+__device__ __forceinline__ int64_t stream_base(const SpmvArgs& a, int c) {
+  const int64_t cb = (int64_t)c * a.chunk;
+  return cb - cb % StreamSlots<BS>::TPL;
+}
+
+template <int BS>
+__device__ __forceinline__ void stream_first(const SpmvArgs& a, int c, int lane, StreamSlots<BS>& s) {
+  const int64_t cb = (int64_t)c * a.chunk;
+  stream_load<BS>(a, stream_base<BS>(a, c), lane, cb, min(cb + (int64_t)a.chunk, a.T), s);
+}
+
+// Variant 2: one wavefront per chunk.  The first step's loads are issued
+// before the plan entry and the rowptr window.
+template <int BS>
+__global__ __launch_bounds__(256) void k_count_stream(const SpmvArgs a) {
+  const int lane = lane_id();
+  const int c = wave_index();
+  if (c >= a.nchunks) return;  // wave-uniform
+  StreamSlots<BS> nxt;
+  stream_first<BS>(a, c, lane, nxt);
+  const int32_t W0 = a.plan[c];
+  int32_t rsl, rel;
+  window_load(a, W0, lane, rsl, rel);
+  stream_chunk<BS>(a, c, nxt, W0, rsl, rel);
+}
+
+// Variant 3, bs 8 only: the streamed count with 32-bit chunk-relative
+// indices and byte-sliced popcounts.  Per step each lane holds 4 tiles
+// (slots u = 2q + e are tiles ps + 128q + 2*lane + e: two 16-byte loads
+// and two 8-byte colind loads per lane, each wave instruction reading one
+// contiguous block).  The 4 x 8 row bytes are transposed with v_perm so
+// that word r holds row r of the 4 tiles; ANDed with the 4 gathered x bytes
+// packed the same way, one v_bcnt per row adds 4 tiles.  A step that lies
+// inside one block row takes this path unmasked; at row boundaries the
+// slots of other rows are zeroed first.  Row totals are reduced with
+// permlane swaps and DPP (no LDS).
+__device__ __forceinline__ void transpose_rows4(uint32_t t0, uint32_t t1, uint32_t t2, uint32_t t3, uint32_t* r) {
+  // v_perm_b32(S0, S1, sel): selector byte k picks byte sel_k of {S0:S1}
+  // (0-3 = S1, 4-7 = S0)
+  const uint32_t u0 = __builtin_amdgcn_perm(t1, t0, 0x05010400u);  // t0.b0 t1.b0 t0.b1 t1.b1
+  const uint32_t u1 = __builtin_amdgcn_perm(t1, t0, 0x07030602u);  // t0.b2 t1.b2 t0.b3 t1.b3
+  const uint32_t u2 = __builtin_amdgcn_perm(t3, t2, 0x05010400u);
+  const uint32_t u3 = __builtin_amdgcn_perm(t3, t2, 0x07030602u);
+  r[0] = __builtin_amdgcn_perm(u2, u0, 0x05040100u);  // t0.b0 t1.b0 t2.b0 t3.b0
+  r[1] = __builtin_amdgcn_perm(u2, u0, 0x07060302u);
+  r[2] = __builtin_amdgcn_perm(u3, u1, 0x05040100u);
+  r[3] = __builtin_amdgcn_perm(u3, u1, 0x07060302u);
+}
+
+// Wave total of row lane / 8 for per-lane row partials v[0..8), in every
+// lane of that row's group of 8 (v is overwritten).  Levels: lanes l and
+// l^32 (permlane32 swap), l and l^16 (permlane16 swap), the two halves of
+// each 16-lane row (DPP row_mirror), then within 8 lanes (DPP).
+__device__ __forceinline__ uint32_t reduce_rows8(uint32_t* v) {
+  const int lane = lane_id();
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    // lanes < 32 end with row k, lanes >= 32 with row k + 4
+    const auto p = __builtin_amdgcn_permlane32_swap(v[k], v[k + 4], false, false);
+    v[k] = p[0] + p[1];
+  }
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    // even 16-lane rows end with k, odd ones with k + 2
+    const auto p = __builtin_amdgcn_permlane16_swap(v[k], v[k + 2], false, false);
+    v[k] = p[0] + p[1];
+  }
+  const bool up = (lane & 8) != 0;
+  const uint32_t keep = up ? v[1] : v[0];
+  const uint32_t send = up ? v[0] : v[1];
+  uint32_t t = keep + (uint32_t)__builtin_amdgcn_update_dpp(0, (int)send, 0x140, 0xF, 0xF, false);  // row_mirror
+  t += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)t, 0x141, 0xF, 0xF, false);  // row_half_mirror
+  t += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)t, 0x4E, 0xF, 0xF, false);   // quad_perm [2,3,0,1]
+  t += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)t, 0xB1, 0xF, 0xF, false);   // quad_perm [1,0,3,2]
+  return t;
+}
+
+struct Slots8 {
+  uint32_t lo[4], hi[4];  // rows 0-3 / 4-7 of the 4 tiles
+  int32_t col[4];
+};
+
+// Loads the step at chunk-relative position ps; positions are relative to
+// the first tile `tl`/`cl` point at, `pend` = end of the chunk's tiles in
+// that frame (groups past it are not read; positions < the chunk start are
+// read and masked later).
+__device__ __forceinline__ void load8(const uint64_t* __restrict__ tl, const int32_t* __restrict__ cl, int32_t ps,
+                                      int32_t pend, int lane, Slots8& s) {
+  typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+  typedef int i32x2 __attribute__((ext_vector_type(2)));
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    const int32_t g = ps + q * 128 + 2 * lane;
+    if (g + 2 <= pend) {
+      const u32x4 v = __builtin_nontemporal_load((const u32x4*)(tl + g));
+      const i32x2 cv = __builtin_nontemporal_load((const i32x2*)(cl + g));
+      s.lo[2 * q] = v.x;
+      s.hi[2 * q] = v.y;
+      s.lo[2 * q + 1] = v.z;
+      s.hi[2 * q + 1] = v.w;
+      s.col[2 * q] = cv.x;
+      s.col[2 * q + 1] = cv.y;
+    } else {
+#pragma unroll
+      for (int e = 0; e < 2; ++e) {
+        const bool ok = g + e < pend;
+        const uint64_t w = ok ? tl[g + e] : 0ull;
+        s.lo[2 * q + e] = (uint32_t)w;
+        s.hi[2 * q + e] = (uint32_t)(w >> 32);
+        s.col[2 * q + e] = ok ? cl[g + e] : 0;
+      }
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_count8(const SpmvArgs a) {
+  const int lane = lane_id();
+  const int c = wave_index();
+  if (c >= a.nchunks) return;  // wave-uniform
+  const int64_t cb = (int64_t)c * a.chunk;
+  const int64_t ce = min(cb + (int64_t)a.chunk, a.T);
+  const bool last = (c == a.nchunks - 1);
+  // frame: positions relative to b0 (a multiple of 4); tiles and rowptr are
+  // int32-indexed (T < 2^31), so every position fits 32 bits
+  const int32_t b0 = (int32_t)(cb & ~(int64_t)3);
+  const int32_t pbeg = (int32_t)(cb - b0), pend = (int32_t)(ce - b0);
+  const uint64_t* __restrict__ tl = (const uint64_t*)a.tiles + b0;
+  const int32_t* __restrict__ cl = a.colind + b0;
+  const uint8_t* __restrict__ xb = (const uint8_t*)a.xbits;
+  int32_t* __restrict__ y = (int32_t*)a.out;
+
+  Slots8 nxt;
+  load8(tl, cl, 0, pend, lane, nxt);
+
+  int32_t W0 = a.plan[c];
+  int32_t rsl, rel;
+  window_load(a, W0, lane, rsl, rel);
+  uint64_t work = 0;
+  bool more = false;
+  auto apply_window = [&]() {
+    const int32_t Rl = W0 + lane;
+    const bool belong = (Rl < a.nbr) && (last || (int64_t)rsl < ce);
+    more = (__ballot(belong) == ~0ull);
+    if (belong && rel == rsl) write_empty_row<8, OP_COUNT>(a, Rl);
+    work = __ballot(belong && rel > rsl);
+  };
+  int32_t R = 0, rs = 0, re = 0;  // current row; [rs, re) clamped to the chunk, frame positions
+  bool full = false;
+  auto next_row = [&]() -> bool {
+    while (!work) {
+      if (!more) return false;
+      W0 += kWave;
+      window_load(a, W0, lane, rsl, rel);
+      apply_window();
+    }
+    const int j = __builtin_ctzll(work);
+    work &= work - 1;
+    R = W0 + j;
+    const int32_t s0 = __builtin_amdgcn_readlane(rsl, j) - b0, e0 = __builtin_amdgcn_readlane(rel, j) - b0;
+    full = (s0 >= pbeg) && (e0 <= pend);
+    rs = s0 > pbeg ? s0 : pbeg;
+    re = e0 < pend ? e0 : pend;
+    return true;
+  };
+
+  uint32_t cnt[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) cnt[k] = 0;
+  auto finish_row = [&]() {
+    const uint32_t tot = reduce_rows8(cnt);
+    const int64_t i = (int64_t)R * 8 + (lane >> 3);
+    if ((lane & 7) == 0 && i < a.nrows) {
+      if (full) {
+        if (a.accumulate) y[i] += (int32_t)tot;
+        else y[i] = (int32_t)tot;
+      } else if (tot) {
+        atomicAdd(&y[i], (int32_t)tot);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) cnt[k] = 0;
+  };
+  // adds the 4 tiles of `s` to the current row: byte u of word r of the
+  // transposed rows is row r of slot u, byte u of xpack is slot u's x word
+  auto add_rows = [&](const uint32_t* lo4, const uint32_t* hi4, uint32_t xpack) {
+    uint32_t r[8];
+    transpose_rows4(lo4[0], lo4[1], lo4[2], lo4[3], r);
+    transpose_rows4(hi4[0], hi4[1], hi4[2], hi4[3], r + 4);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) cnt[k] += (uint32_t)__builtin_popcount(r[k] & xpack);
+  };
+
+  apply_window();
+  bool open = next_row();  // a row is being accumulated
+  for (int32_t ps = 0; open && ps < pend; ps += 256) {
+    const Slots8 cur = nxt;
+    if (ps + 256 < pend) load8(tl, cl, ps + 256, pend, lane, nxt);
+    const uint32_t x0 = xb[(uint32_t)cur.col[0]], x1 = xb[(uint32_t)cur.col[1]];
+    const uint32_t x2 = xb[(uint32_t)cur.col[2]], x3 = xb[(uint32_t)cur.col[3]];
+    const uint32_t xpack = x0 | (x1 << 8) | (x2 << 16) | (x3 << 24);
+    const int32_t tend = ps + 256 < pend ? ps + 256 : pend;
+    while (true) {
+      const int32_t lo = rs > ps ? rs : ps, hi = re < tend ? re : tend;
+      if (lo == ps && hi == ps + 256) {
+        add_rows(cur.lo, cur.hi, xpack);  // the whole step is in this row
+      } else {
+        uint32_t lo4[4], hi4[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int32_t p = ps + (u >> 1) * 128 + 2 * lane + (u & 1);
+          const bool in = (p >= lo) && (p < hi);
+          lo4[u] = in ? cur.lo[u] : 0u;
+          hi4[u] = in ? cur.hi[u] : 0u;
+        }
+        add_rows(lo4, hi4, xpack);
+      }
+      if (re > tend) break;  // the row goes on past this step
+      finish_row();
+      open = next_row();
+      if (!open || rs >= tend) break;
     }
   }
   if (open) finish_row();  // a row that continues past the chunk: atomic partial
@@ -767,10 +995,13 @@ __global__ __launch_bounds__(256) void k_sum_fixup(const SpmvArgs a) {
 
 template <int BS, int OP>
 static int spmv_launch_t(const SpmvArgs& a, hipStream_t st) {
-  // variant 2 (streamed) reads 16- or 8-byte vectors of tiles and column
-  // indices: it needs 16-byte-aligned arrays, otherwise variant 0 runs.
+  // variants 2 and 3 (streamed) read 16- or 8-byte vectors of tiles and
+  // column indices: they need 16-byte-aligned arrays, otherwise variant 0
+  // runs.
   const bool aligned = ((((uintptr_t)a.tiles) | ((uintptr_t)a.colind)) & 15) == 0;
-  const bool stream = (BS <= 16 && OP == OP_COUNT && a.variant == 2 && aligned);
+  // variant 3 (k_count8) exists for bs 8; bs 16 runs variant 2 for it
+  const bool lean8 = (BS == 8 && OP == OP_COUNT && a.variant == 3 && aligned);
+  const bool stream = (BS <= 16 && OP == OP_COUNT && (a.variant == 2 || a.variant == 3) && aligned && !lean8);
   const bool lpt = (BS <= 16 && (OP == OP_COUNT || OP == OP_BFS) && a.variant == 1);
   const bool need_prefill = OP == OP_BFS || OP == OP_MINPLUS || OP == OP_MIN || (OP == OP_COUNT && !a.accumulate);
   if (need_prefill && a.nchunks > 1) {
@@ -780,7 +1011,9 @@ static int spmv_launch_t(const SpmvArgs& a, hipStream_t st) {
   }
   const int64_t blocks = ((int64_t)a.nchunks + 3) / 4;
   if constexpr (BS <= 16 && OP == OP_COUNT) {
-    if (stream) {
+    if (lean8) {
+      if constexpr (BS == 8) k_count8<<<(unsigned)blocks, 256, 0, st>>>(a);
+    } else if (stream) {
       k_count_stream<BS><<<(unsigned)blocks, 256, 0, st>>>(a);
     } else if (lpt) {
       k_spmv_lpt<BS, OP><<<(unsigned)blocks, 256, 0, st>>>(a);

@@ -548,7 +548,7 @@ int gbo_bsr_plan_ex(gbo_bsr* A, int32_t chunk_tiles, int32_t variant, gbo_stream
   int rc = validate_bsr(A, -1, "gbo_bsr_plan");
   if (rc) return rc;
   if (chunk_tiles < 0) return set_error(GBO_EINVAL, "gbo_bsr_plan: negative chunk");
-  if (variant < 0 || variant > 2) return set_error(GBO_EINVAL, "gbo_bsr_plan: variant must be 0, 1 or 2");
+  if (variant < 0 || variant > 3) return set_error(GBO_EINVAL, "gbo_bsr_plan: variant must be 0, 1, 2 or 3");
   const int32_t chunk = chunk_tiles > 0 ? chunk_tiles : default_chunk(A->bs);
   const int64_t nch = std::max<int64_t>(1, (A->nblocks + chunk - 1) / chunk);
   gbo_bsr_plan_free(A);

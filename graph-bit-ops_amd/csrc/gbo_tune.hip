@@ -80,7 +80,7 @@ int gbo_autotune(const int32_t* d_rowptr, const int32_t* d_colind, int32_t nrows
     GBO_HIP_CHECK(hipMemsetAsync(vis, 0, nw * bs / 8, st));
     const int dc = default_chunk(bs);
     const int chunks[3] = {dc, dc * 2, dc / 2};
-    const int nvar = bs <= 16 ? (op == 0 ? 3 : 2) : 1;
+    const int nvar = bs == 8 ? (op == 0 ? 4 : 2) : bs == 16 ? (op == 0 ? 3 : 2) : 1;
     for (int v = 0; v < nvar && rc == GBO_OK; ++v)
       for (int ci = 0; ci < 3 && rc == GBO_OK; ++ci) {
         rc = gbo_bsr_plan_ex(&A, chunks[ci], v, st);

@@ -180,7 +180,10 @@ int gbo_bsr_plan(gbo_bsr* A, int32_t chunk_tiles, gbo_stream_t stream);
  * 1 = one lane per tile, block rows one after another, 2 = streamed: each
  * lane reads 4 (bs 8) or 2 (bs 16) consecutive tiles per step with the next
  * step's loads in flight across block-row boundaries (COUNT only; BFS runs
- * variant 0; needs 16-byte-aligned tiles/colind, else variant 0 runs).
+ * variant 0; needs 16-byte-aligned tiles/colind, else variant 0 runs),
+ * 3 = bs 8 COUNT with 32-bit chunk-relative indexing and byte-transposed
+ * popcounts (4 tiles per v_bcnt) and a permlane/DPP row reduction (bs 16
+ * runs variant 2).
  * Results are identical; only speed differs (the autotuner picks). */
 int gbo_bsr_plan_ex(gbo_bsr* A, int32_t chunk_tiles, int32_t variant, gbo_stream_t stream);
 void gbo_bsr_plan_free(gbo_bsr* A);

@@ -30,7 +30,7 @@ def packed(gbo, g, bs, chunk, variant=0):
 
 @pytest.mark.parametrize("bs", BS)
 @pytest.mark.parametrize("chunk", COUNT_CHUNKS)
-@pytest.mark.parametrize("variant", (0, 1, 2))
+@pytest.mark.parametrize("variant", (0, 1, 2, 3))
 def test_count(gbo, po, graph_cases, bs, chunk, variant):
     if variant and bs > 16:
         pytest.skip("kernel variants exist for bs 8/16 only")
@@ -46,7 +46,7 @@ def test_count(gbo, po, graph_cases, bs, chunk, variant):
 
 
 @pytest.mark.parametrize("bs", (8, 16, 32))
-@pytest.mark.parametrize("variant", (0, 1, 2))
+@pytest.mark.parametrize("variant", (0, 1, 2, 3))
 def test_count_long_rows(gbo, po, bs, variant):
     """Block rows of >300k tiles, each row count > 2^16, in one chunk: the
     per-lane 16-bit field accumulators of the bs 8/16 variants must be
