@@ -756,14 +756,13 @@ __global__ __launch_bounds__(256) void k_count_stream(const SpmvArgs a) {
 
 // Variant 3, bs 8 only: the streamed count with 32-bit chunk-relative
 // indices and byte-sliced popcounts.  Per step each lane holds 4 tiles
-// (slots u = 2q + e are tiles ps + 128q + 2*lane + e: two 16-byte loads
-// and two 8-byte colind loads per lane, each wave instruction reading one
-// contiguous block).  The 4 x 8 row bytes are transposed with v_perm so
-// that word r holds row r of the 4 tiles; ANDed with the 4 gathered x bytes
-// packed the same way, one v_bcnt per row adds 4 tiles.  A step that lies
-// inside one block row takes this path unmasked; at row boundaries the
-// slots of other rows are zeroed first.  Row totals are reduced with
-// permlane swaps and DPP (no LDS).
+// (slots u = 2q + e are tiles ps + 128q + 2*lane + e: 16-byte loads, each
+// wave instruction reading one contiguous block).  The 4 x 8 row bytes are
+// transposed with v_perm so that word r holds row r of the 4 tiles; ANDed
+// with the 4 gathered x bytes packed the same way, one v_bcnt per row adds
+// 4 tiles.  A step that lies inside one block row takes this path
+// unmasked; at row boundaries the slots of other rows are zeroed first.
+// Row totals are reduced with permlane swaps and DPP (no LDS).
 __device__ __forceinline__ void transpose_rows4(uint32_t t0, uint32_t t1, uint32_t t2, uint32_t t3, uint32_t* r) {
   // v_perm_b32(S0, S1, sel): selector byte k picks byte sel_k of {S0:S1}
   // (0-3 = S1, 4-7 = S0)
@@ -805,21 +804,27 @@ __device__ __forceinline__ uint32_t reduce_rows8(uint32_t* v) {
   return t;
 }
 
+// NQ vector groups of 2 consecutive tiles per lane per step: slots u = 2q + e
+// are tiles ps + 128q + 2*lane + e (one 16-byte tile load and one 8-byte
+// colind load per group; each wave instruction reads one contiguous block),
+// a step is 128*NQ tiles.
+template <int NQ>
 struct Slots8 {
-  uint32_t lo[4], hi[4];  // rows 0-3 / 4-7 of the 4 tiles
-  int32_t col[4];
+  uint32_t lo[2 * NQ], hi[2 * NQ];  // rows 0-3 / 4-7 of each tile
+  int32_t col[2 * NQ];
 };
 
 // Loads the step at chunk-relative position ps; positions are relative to
 // the first tile `tl`/`cl` point at, `pend` = end of the chunk's tiles in
-// that frame (groups past it are not read; positions < the chunk start are
-// read and masked later).
+// that frame (groups past it are not read; positions before the chunk start
+// are read and masked later).
+template <int NQ>
 __device__ __forceinline__ void load8(const uint64_t* __restrict__ tl, const int32_t* __restrict__ cl, int32_t ps,
-                                      int32_t pend, int lane, Slots8& s) {
+                                      int32_t pend, int lane, Slots8<NQ>& s) {
   typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
   typedef int i32x2 __attribute__((ext_vector_type(2)));
 #pragma unroll
-  for (int q = 0; q < 2; ++q) {
+  for (int q = 0; q < NQ; ++q) {
     const int32_t g = ps + q * 128 + 2 * lane;
     if (g + 2 <= pend) {
       const u32x4 v = __builtin_nontemporal_load((const u32x4*)(tl + g));
@@ -843,7 +848,13 @@ __device__ __forceinline__ void load8(const uint64_t* __restrict__ tl, const int
   }
 }
 
+// Steps are processed one after another, without a prefetch buffer: with
+// the default chunk of one step, a wave issues all of its tile and colind
+// loads before the plan entry and the rowptr window, and the registers a
+// second buffer would take go to occupancy instead.
+template <int NQ>
 __global__ __launch_bounds__(256) void k_count8(const SpmvArgs a) {
+  constexpr int32_t STEP = 128 * NQ;
   const int lane = lane_id();
   const int c = wave_index();
   if (c >= a.nchunks) return;  // wave-uniform
@@ -859,8 +870,8 @@ __global__ __launch_bounds__(256) void k_count8(const SpmvArgs a) {
   const uint8_t* __restrict__ xb = (const uint8_t*)a.xbits;
   int32_t* __restrict__ y = (int32_t*)a.out;
 
-  Slots8 nxt;
-  load8(tl, cl, 0, pend, lane, nxt);
+  Slots8<NQ> cur;
+  load8<NQ>(tl, cl, 0, pend, lane, cur);
 
   int32_t W0 = a.plan[c];
   int32_t rsl, rel;
@@ -910,7 +921,7 @@ __global__ __launch_bounds__(256) void k_count8(const SpmvArgs a) {
 #pragma unroll
     for (int k = 0; k < 8; ++k) cnt[k] = 0;
   };
-  // adds the 4 tiles of `s` to the current row: byte u of word r of the
+  // adds groups of 4 slots to the current row: byte u of word r of the
   // transposed rows is row r of slot u, byte u of xpack is slot u's x word
   auto add_rows = [&](const uint32_t* lo4, const uint32_t* hi4, uint32_t xpack) {
     uint32_t r[8];
@@ -922,27 +933,35 @@ __global__ __launch_bounds__(256) void k_count8(const SpmvArgs a) {
 
   apply_window();
   bool open = next_row();  // a row is being accumulated
-  for (int32_t ps = 0; open && ps < pend; ps += 256) {
-    const Slots8 cur = nxt;
-    if (ps + 256 < pend) load8(tl, cl, ps + 256, pend, lane, nxt);
-    const uint32_t x0 = xb[(uint32_t)cur.col[0]], x1 = xb[(uint32_t)cur.col[1]];
-    const uint32_t x2 = xb[(uint32_t)cur.col[2]], x3 = xb[(uint32_t)cur.col[3]];
-    const uint32_t xpack = x0 | (x1 << 8) | (x2 << 16) | (x3 << 24);
-    const int32_t tend = ps + 256 < pend ? ps + 256 : pend;
+  for (int32_t ps = 0; open && ps < pend; ps += STEP) {
+    if (ps > 0) load8<NQ>(tl, cl, ps, pend, lane, cur);
+    uint32_t xpack[NQ / 2];
+#pragma unroll
+    for (int h = 0; h < NQ / 2; ++h) {
+      const uint32_t x0 = xb[(uint32_t)cur.col[4 * h]], x1 = xb[(uint32_t)cur.col[4 * h + 1]];
+      const uint32_t x2 = xb[(uint32_t)cur.col[4 * h + 2]], x3 = xb[(uint32_t)cur.col[4 * h + 3]];
+      xpack[h] = x0 | (x1 << 8) | (x2 << 16) | (x3 << 24);
+    }
+    const int32_t tend = ps + STEP < pend ? ps + STEP : pend;
     while (true) {
       const int32_t lo = rs > ps ? rs : ps, hi = re < tend ? re : tend;
-      if (lo == ps && hi == ps + 256) {
-        add_rows(cur.lo, cur.hi, xpack);  // the whole step is in this row
-      } else {
-        uint32_t lo4[4], hi4[4];
+      if (lo == ps && hi == ps + STEP) {
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const int32_t p = ps + (u >> 1) * 128 + 2 * lane + (u & 1);
-          const bool in = (p >= lo) && (p < hi);
-          lo4[u] = in ? cur.lo[u] : 0u;
-          hi4[u] = in ? cur.hi[u] : 0u;
+        for (int h = 0; h < NQ / 2; ++h) add_rows(cur.lo + 4 * h, cur.hi + 4 * h, xpack[h]);  // whole step in this row
+      } else {
+#pragma unroll
+        for (int h = 0; h < NQ / 2; ++h) {
+          uint32_t lo4[4], hi4[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const int v = 4 * h + u;
+            const int32_t p = ps + (v >> 1) * 128 + 2 * lane + (v & 1);
+            const bool in = (p >= lo) && (p < hi);
+            lo4[u] = in ? cur.lo[v] : 0u;
+            hi4[u] = in ? cur.hi[v] : 0u;
+          }
+          add_rows(lo4, hi4, xpack[h]);
         }
-        add_rows(lo4, hi4, xpack);
       }
       if (re > tend) break;  // the row goes on past this step
       finish_row();
@@ -1012,7 +1031,11 @@ static int spmv_launch_t(const SpmvArgs& a, hipStream_t st) {
   const int64_t blocks = ((int64_t)a.nchunks + 3) / 4;
   if constexpr (BS <= 16 && OP == OP_COUNT) {
     if (lean8) {
-      if constexpr (BS == 8) k_count8<<<(unsigned)blocks, 256, 0, st>>>(a);
+      // steps of 512 tiles for chunks of 512 or more, else 256
+      if constexpr (BS == 8) {
+        if (a.chunk >= 512) k_count8<4><<<(unsigned)blocks, 256, 0, st>>>(a);
+        else k_count8<2><<<(unsigned)blocks, 256, 0, st>>>(a);
+      }
     } else if (stream) {
       k_count_stream<BS><<<(unsigned)blocks, 256, 0, st>>>(a);
     } else if (lpt) {
