@@ -13,11 +13,16 @@
 // The product is a bit-BSR SpMM with a dense K-column right-hand side in
 // vertex-major layout X[n][K]: lane k of a wave owns column k, so every set
 // bit (r, c) of a tile costs one coalesced 4K-byte row read X[j][0..K).
-// Tile row words are fetched by lanes r < bs, the rows with bits are found
-// by a ballot, and each row's partial sum lives in register acc[r]
-// (uniform index).  Waves take tile-balanced chunks; rows cut by a chunk
-// boundary leave compensated partials that a fix-up kernel adds in chunk
-// order, so results are bitwise reproducible.
+// At bs 8 (k_ppr_spmm8) the set bits are listed in LDS by all lanes and
+// read back 8 at a time, so 8 row reads are in flight per wave; X carries
+// the per-source scale (one elementwise pass).  The row reads are random
+// 256-byte accesses into an n x 256 B matrix: on MI355X such reads top out
+// near 2 TB/s, which bounds this product.  At bs 16/32 (k_ppr_spmm) tile
+// row words are fetched by lanes r < bs, the rows with bits are found by a
+// ballot and set bits are read one at a time.  Each row's partial sum lives
+// in register acc[r] (uniform index).  Waves take tile-balanced chunks;
+// rows cut by a chunk boundary leave partials that a fix-up kernel adds in
+// chunk order, so results are bitwise reproducible.
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -46,6 +51,12 @@ struct PprArgs {
   float* Y;
   float* partial;            // [nchunks][2][bs][64] split-row partials (sum + compensation)
 };
+
+// First tile of chunk c (c <= nchunks; chunk nchunks starts at T).
+__device__ __forceinline__ int64_t chunk_start(const PprArgs& A, int64_t c) {
+  const int64_t b = c * A.chunk;
+  return b < A.T ? b : A.T;
+}
 
 __device__ __forceinline__ void nadd(float& s, float& c, float v) {
   const float t = s + v;
@@ -80,8 +91,8 @@ __global__ __launch_bounds__(256) void k_ppr_spmm(const PprArgs A) {
   const int lane = lane_id();
   const int c = wave_index();
   if (c >= A.nchunks) return;
-  const int64_t cb = (int64_t)c * A.chunk;
-  const int64_t ce = min(cb + (int64_t)A.chunk, A.T);
+  const int64_t cb = chunk_start(A, c);
+  const int64_t ce = chunk_start(A, c + 1);
   const bool last = (c == A.nchunks - 1);
   const W* __restrict__ tiles = (const W*)A.tiles;
   const bool col_ok = lane < A.K;
@@ -132,6 +143,121 @@ __global__ __launch_bounds__(256) void k_ppr_spmm(const PprArgs A) {
   }
 }
 
+// bs 8 SpMM with batched row gathers.  Lanes are columns, so every set bit
+// (r, c) of a tile is one coalesced row read X[v][0..K) for the whole wave.
+// The bits are listed in parallel: lane i takes tile b + i of a 64-tile
+// block of the row segment, a wave prefix sum of the tiles' popcounts gives
+// each bit its slot, and each lane writes (r << 29 | v) for its bits into a
+// per-wave LDS list (in tile order, then bit order, so the summation order
+// is fixed).  The wave then walks the list 8 entries at a time, issuing the
+// 8 row reads together.  Sums are FP64, one register per tile row indexed
+// by the wave-uniform row.  X must already carry any per-source scale.
+constexpr int kPprCap = 1024;  // list entries per wave and pass
+
+__global__ __launch_bounds__(256) void k_ppr_spmm8(const PprArgs A) {
+  constexpr int BS = 8, NB = 8;
+  __shared__ uint32_t lists[256 / kWave][kPprCap];
+  const int lane = lane_id();
+  uint32_t* __restrict__ list = lists[threadIdx.x / kWave];
+  const int c = wave_index();
+  if (c >= A.nchunks) return;
+  const int64_t cb = chunk_start(A, c);
+  const int64_t ce = chunk_start(A, c + 1);
+  const bool last = (c == A.nchunks - 1);
+  const uint64_t* __restrict__ tw = (const uint64_t*)A.tiles;
+  const bool col_ok = lane < A.K;
+
+  for (int32_t R0 = A.plan[c]; R0 < A.nbr;) {
+    const int32_t Rl = R0 + lane;
+    int32_t rsl = 0, rel = 0;
+    if (Rl < A.nbr) {
+      rsl = A.rowptr[Rl];
+      rel = A.rowptr[Rl + 1];
+    }
+    const bool belong = (Rl < A.nbr) && (last || (int64_t)rsl < ce);
+    const uint64_t bmask = __ballot(belong);
+    uint64_t todo = bmask;  // empty rows are written with the epilogue too
+    while (todo) {
+      const int j = __builtin_ctzll(todo);
+      todo &= todo - 1;
+      const int64_t R = R0 + j;
+      const int64_t rs = __builtin_amdgcn_readlane(rsl, j), re = __builtin_amdgcn_readlane(rel, j);
+      const int64_t s = rs > cb ? rs : cb;
+      const int64_t te = re < ce ? re : ce;
+      const bool full = (rs >= cb) && (re <= ce);
+      double acc[BS];
+#pragma unroll
+      for (int r = 0; r < BS; ++r) acc[r] = 0.0;
+      for (int64_t b = s; b < te; b += kWave) {
+        const int64_t nt = min((int64_t)kWave, te - b);
+        const uint64_t word = lane < nt ? tw[b + lane] : 0ull;
+        const uint32_t vb = lane < nt ? (uint32_t)A.colind[b + lane] * BS + (BS - 1) : 0u;
+        const int cnt = __popcll(word);
+        int incl = cnt;  // inclusive prefix sum of the popcounts over the lanes
+#pragma unroll
+        for (int off = 1; off < kWave; off <<= 1) {
+          const int o = __shfl_up(incl, off);
+          if (lane >= off) incl += o;
+        }
+        const int excl = incl - cnt;
+        const int tot = __builtin_amdgcn_readlane(incl, kWave - 1);
+        for (int base = 0; base < tot; base += kPprCap) {
+          // list entries [base, base + kPprCap) of this block
+          uint64_t w = word;
+          int idx = excl;
+          while (w) {
+            const int p = __builtin_ctzll(w);
+            w &= w - 1;
+            if (idx >= base && idx < base + kPprCap)
+              list[idx - base] = ((uint32_t)(p >> 3) << 29) | (vb - (uint32_t)(p & 7));  // row p/8, column 7 - p%8
+            ++idx;
+          }
+          __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+          const int nl = min(kPprCap, tot - base);
+          for (int i = 0; i < nl; i += NB) {
+            uint32_t e[NB];
+#pragma unroll
+            for (int k = 0; k < NB; ++k) e[k] = __builtin_amdgcn_readfirstlane(list[min(i + k, nl - 1)]);
+            float xk[NB];
+#pragma unroll
+            for (int k = 0; k < NB; ++k)
+              xk[k] = (i + k < nl && col_ok) ? A.X[(int64_t)(e[k] & 0x1FFFFFFFu) * A.ld + lane] : 0.0f;
+#pragma unroll
+            for (int k = 0; k < NB; ++k)
+              if (i + k < nl) acc[e[k] >> 29] += (double)xk[k];
+          }
+          __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        }
+      }
+      for (int r = 0; r < BS; ++r) {
+        const int64_t i = R * BS + r;
+        if (i >= A.nrows) break;
+        if (full) {
+          if (col_ok) {
+            const double rsc = A.rowscale ? (double)A.rowscale[i] : 1.0;
+            double y = (double)A.a * rsc * acc[r];
+            if (A.Z) y += (double)A.b * (double)A.Z[i * A.ld + lane];
+            A.Y[i * A.ld + lane] = (float)y;
+          }
+        } else {
+          A.partial[(((int64_t)c * 2 + ((rs < cb) ? 0 : 1)) * BS + r) * kWave + lane] = (float)acc[r];
+        }
+      }
+    }
+    if (__popcll(bmask) < kWave) break;
+    R0 += kWave;
+  }
+}
+
+// Xs[i][k] = X[i][k] * scale[i] over K columns of a row stride ld.
+__global__ __launch_bounds__(256) void k_row_scale(const float* __restrict__ X, const float* __restrict__ scale,
+                                                   float* __restrict__ Xs, int32_t n, int32_t K, int32_t ld) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)n * K) return;
+  const int64_t i = idx / K, k = idx % K;
+  Xs[i * ld + k] = X[i * ld + k] * scale[i];
+}
+
 template <int BS>
 __global__ __launch_bounds__(256) void k_ppr_fixup(const PprArgs A) {
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -140,12 +266,12 @@ __global__ __launch_bounds__(256) void k_ppr_fixup(const PprArgs A) {
   const int64_t c = cr / BS;
   const int r = (int)(cr % BS);
   if (c + 1 >= A.nchunks) return;
-  const int64_t nb = (c + 1) * A.chunk;
+  const int64_t nb = chunk_start(A, c + 1);
   const int32_t R = A.plan[c + 1];
   const int64_t rs = A.rowptr[R], re = A.rowptr[R + 1];
-  if (!(rs < nb) || rs < c * A.chunk) return;
+  if (!(rs < nb) || rs < chunk_start(A, c)) return;
   float s = A.partial[((c * 2 + 1) * BS + r) * kWave + lane], cmp = 0.0f;
-  for (int64_t c2 = c + 1; c2 < A.nchunks && c2 * A.chunk < re; ++c2)
+  for (int64_t c2 = c + 1; c2 < A.nchunks && chunk_start(A, c2) < re; ++c2)
     nadd(s, cmp, A.partial[((c2 * 2 + 0) * BS + r) * kWave + lane]);
   const int64_t i = (int64_t)R * BS + r;
   if (i < A.nrows && lane < A.K) {
@@ -204,6 +330,8 @@ struct PprPlan {
   int32_t* plan = nullptr;
   int32_t chunk = 0, nchunks = 0;
   float* partial = nullptr;
+  float* xs = nullptr;  // bs 8: X with the per-source scale applied, [n][K]
+  int64_t xs_len = 0;
 };
 
 __global__ void k_plan_ppr(const int32_t* __restrict__ rowptr, int32_t nbr, int32_t chunk, int32_t nchunks,
@@ -237,13 +365,34 @@ static int make_ppr_plan(const gbo_bsr* M, PprPlan* P, hipStream_t st) {
 static void free_ppr_plan(PprPlan* P, hipStream_t st) {
   if (P->plan) (void)hipFreeAsync(P->plan, st);
   if (P->partial) (void)hipFreeAsync(P->partial, st);
+  if (P->xs) (void)hipFreeAsync(P->xs, st);
   P->plan = nullptr;
   P->partial = nullptr;
+  P->xs = nullptr;
+  P->xs_len = 0;
 }
 
 // Y = a * rowscale * (M (colscale .* X)) + b * Z over all K columns.
-static int spmm(const gbo_bsr* M, const PprPlan& P, int32_t K, const float* X, const float* colscale,
+static int spmm(const gbo_bsr* M, PprPlan& P, int32_t K, const float* X, const float* colscale,
                 const float* rowscale, float a, float b, const float* Z, float* Y, hipStream_t st) {
+  if (M->bs == 8 && M->ncols > (1 << 29))
+    return set_error(GBO_EUNSUPPORTED, "PPR at block size 8 supports up to 2^29 vertices");
+  if (M->bs == 8 && colscale) {
+    // the bs 8 kernel reads pre-scaled rows
+    const int64_t len = (int64_t)M->ncols * K;
+    if (P.xs_len < len) {
+      if (P.xs) GBO_HIP_CHECK(hipFreeAsync(P.xs, st));
+      P.xs = nullptr;
+      GBO_HIP_CHECK(hipMallocAsync((void**)&P.xs, sizeof(float) * std::max<int64_t>(len, 1), st));
+      P.xs_len = len;
+    }
+    if (len > 0) {
+      k_row_scale<<<(unsigned)((len + 255) / 256), 256, 0, st>>>(X, colscale, P.xs, M->ncols, K, K);
+      GBO_LAUNCH_CHECK();
+    }
+    X = P.xs;
+    colscale = nullptr;
+  }
   for (int32_t k0 = 0; k0 < K; k0 += kWave) {
     PprArgs A{};
     A.rowptr = M->rowptr;
@@ -267,7 +416,7 @@ static int spmm(const gbo_bsr* M, const PprPlan& P, int32_t K, const float* X, c
     A.partial = P.partial;
     const unsigned blocks = (unsigned)(((int64_t)P.nchunks + 3) / 4);
     switch (M->bs) {
-      case 8: k_ppr_spmm<8><<<blocks, 256, 0, st>>>(A); break;
+      case 8: k_ppr_spmm8<<<blocks, 256, 0, st>>>(A); break;
       case 16: k_ppr_spmm<16><<<blocks, 256, 0, st>>>(A); break;
       case 32: k_ppr_spmm<32><<<blocks, 256, 0, st>>>(A); break;
       default: return set_error(GBO_EUNSUPPORTED, "PPR supports block sizes 8, 16, 32");
