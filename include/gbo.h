@@ -184,7 +184,8 @@ int gbo_bsr_plan(gbo_bsr* A, int32_t chunk_tiles, gbo_stream_t stream);
  * 3 = bs 8 COUNT with 32-bit chunk-relative indexing and byte-transposed
  * popcounts (4 tiles per v_bcnt) and a permlane/DPP row reduction (bs 16
  * runs variant 2).
- * Results are identical; only speed differs (the autotuner picks). */
+ * Results are identical; only speed differs (the autotuner picks).
+ * gbo_bsr_create / gbo_bsr_create_dev set variant 3 for bs 8, 0 otherwise. */
 int gbo_bsr_plan_ex(gbo_bsr* A, int32_t chunk_tiles, int32_t variant, gbo_stream_t stream);
 void gbo_bsr_plan_free(gbo_bsr* A);
 /* Per-nonzero values in tile bit order (row-major, MSB-first inside a row
