@@ -116,6 +116,47 @@ def time_bfs_step(gbo, A, dx, steps, warmup, timer):
     return ms
 
 
+def time_value_sweeps(gbo, A, g, w_sssp, steps, warmup, timer):
+    """One full sweep of each value op over every tile: sum (PageRank's
+    product, x = 1/n), min (CC's product, x = vertex ids) and min-plus
+    (SSSP's product with the uint32 [1,64] weights, half the distances
+    finite).  Algorithmic bytes: tiles + colind + rowptr + 4n in + 4n out,
+    and for min-plus also 4 bytes per nonzero of weights + 8 per tile of
+    weight offsets."""
+    L = gbo.lib()
+    n = A.nrows
+    base = sweep_bytes(A, n) - (n + 7) // 8 - 4 * n  # tiles, colind, rowptr
+    out = {}
+    dy = gbo.DeviceBuffer(4 * max(n, 1))
+    rng = np.random.default_rng(7)
+    xs = gbo.DeviceBuffer.from_numpy(np.full(n, 1.0 / n, np.float32))
+    xi = gbo.DeviceBuffer.from_numpy(np.arange(n, dtype=np.int32))
+    dist = rng.integers(0, 1000, n).astype(np.uint32)
+    dist[rng.random(n) < 0.5] = 0xFFFFFFFF
+    xd = gbo.DeviceBuffer.from_numpy(dist)
+    tv = gbo.TileValues(A, g, w_sssp)
+    calls = {
+        "sum_f32": (lambda: L.gbo_bsrbmv_sum_f32(A.ref, xs.ptr, dy.ptr, None), base + 8 * n),
+        "min_i32": (lambda: L.gbo_bsrbmv_min_i32(A.ref, xi.ptr, dy.ptr, None), base + 8 * n),
+        "minplus_weighted": (lambda: L.gbo_bsrbmv_minplus(A.ref, tv.tval.ptr, tv.woff.ptr, xd.ptr, dy.ptr, None),
+                             base + 8 * n + 4 * g.nnz + 8 * A.nblocks),
+    }
+    for name, (fn, byt) in calls.items():
+        for _ in range(warmup):
+            gbo._check(fn())
+        gbo.synchronize()
+        timer.start()
+        for _ in range(steps):
+            gbo._check(fn())
+        ms = timer.stop_ms() / steps
+        out[name] = {"ms": round(ms, 4), "bytes": int(byt),
+                     "frac_hbm": round(byt / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    tv.free()
+    for b in (dy, xs, xi, xd):
+        b.free()
+    return out
+
+
 def copy_ceiling(gbo, timer, nbytes=1 << 30, reps=10):
     """HBM copy ceiling (read + write bytes / time) of a plain copy kernel."""
     L = gbo.lib()
@@ -418,6 +459,7 @@ def main():
                            "reached": int((dist != 0xFFFFFFFF).sum()), "ms": round(sssp_s * 1e3, 1)}
             cc_s, (lab, cit) = timed(lambda: gbo.cc(A))
             rec["cc"] = {"iters": cit, "components": int(len(np.unique(lab))), "ms": round(cc_s * 1e3, 1)}
+            rec["value_sweeps"] = time_value_sweeps(gbo, A, g, w_sssp, max(3, args.steps // 4), 1, timer)
             log(f"[bench] bs={bs} bfs {rec['bfs']} pr {rec['pagerank']} sssp {rec['sssp']} cc {rec['cc']}")
         per_bs[bs] = rec
         keep[bs] = (A, dx)

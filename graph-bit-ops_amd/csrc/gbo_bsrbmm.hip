@@ -10,8 +10,10 @@
 // contribute to mask tile (I,K) are the intersection of two sorted lists:
 // block row I of A and block column K of B.  Each wave takes a fixed chunk
 // of mask tiles; its 64 lanes search the longer list for 64 entries of the
-// shorter one at a time, and for every match lane a (tile row) adds
-// popc(A_IJ[a] & B_JK[k]) for the columns k set in mask row a.
+// shorter one at a time, and each lane that finds a match J adds that
+// term itself: with B's tiles turned into row form once per call,
+// sum_{a,k} M[a][k] (A_IJ B_JK)[a][k] = sum over the set bits (a,j) of A_IJ
+// of popc(M[a] & B_JK row j) -- one popcount per set bit of A_IJ.
 //
 // Plain product sum (bmm{N}_sparse, bsrbmm/bsrbmm.cu:296-555):
 //   sum_{i,k} (A B)_ik = sum_j colsum_A(j) * rowsum_B(j),
@@ -49,6 +51,60 @@ __device__ __forceinline__ int64_t lower_bound_i32(const int32_t* __restrict__ a
   return lo;
 }
 
+// Row-packed copy of column-packed tiles (same positions): word j of the
+// output holds row j of the tile, bit BS-1-k = column k.  One wave takes
+// 64/BS tiles; lane l holds column word l % BS of tile l / BS, and one
+// ballot per row j collects that row over the columns of every tile.
+template <int BS>
+__global__ __launch_bounds__(256) void k_tile_rows(const word_t<BS>* __restrict__ in, int64_t T,
+                                                   word_t<BS>* __restrict__ out) {
+  constexpr int G = kWave / BS;
+  const int lane = lane_id();
+  const int64_t t = (int64_t)wave_index() * G + lane / BS;  // tile of this lane
+  const int k = lane % BS;                                  // its column word
+  const uint64_t w = t < T ? (uint64_t)in[t * BS + k] : 0ull;
+  const int g = lane / BS;
+#pragma unroll 8
+  for (int j = 0; j < BS; ++j) {
+    const uint64_t b = __ballot((w >> (BS - 1 - j)) & 1ull);  // bit l: column l % BS of tile l / BS has row j
+    const uint64_t mine = (b >> (g * BS)) & low_mask<BS>();
+    if (k == j && t < T) out[t * BS + j] = (word_t<BS>)brev_low<BS>(mine);
+  }
+}
+
+// Triangles contributed by one matching (I,J,K): with A = tile (I,J) of L,
+// M = mask tile (I,K) and R = B's tile (J,K) in row form,
+//   sum_{a,k} M[a][k] sum_j A[a][j] R[j][k] = sum_{(a,j) in A} popc(M[a] & R[j]),
+// one popcount per set bit of A.
+template <int BS>
+__device__ __forceinline__ uint64_t tc_hit(const word_t<BS>* __restrict__ A, const word_t<BS>* __restrict__ M,
+                                           const word_t<BS>* __restrict__ R) {
+  uint64_t s = 0;
+  if constexpr (BS == 8) {
+    uint64_t a = *(const uint64_t*)A;  // byte r = row r, bit 7 - c = column c
+    const uint64_t m = *(const uint64_t*)M, r = *(const uint64_t*)R;
+    while (a) {
+      const int p = __builtin_ctzll(a);
+      a &= a - 1;
+      const int row = p >> 3, col = 7 - (p & 7);
+      s += __popc((uint32_t)((m >> (8 * row)) & (r >> (8 * col)) & 0xFFu));
+    }
+  } else {
+    for (int row = 0; row < BS; ++row) {
+      uint64_t wa = (uint64_t)A[row];
+      if (!wa) continue;
+      const uint64_t ma = (uint64_t)M[row];
+      if (!ma) continue;
+      while (wa) {
+        const int p = __builtin_ctzll(wa);
+        wa &= wa - 1;
+        s += (uint64_t)popcw((word_t<BS>)(ma & (uint64_t)R[BS - 1 - p]));
+      }
+    }
+  }
+  return s;
+}
+
 template <int BS>
 __global__ __launch_bounds__(256) void k_tc(const TcArgs a) {
   using W = word_t<BS>;
@@ -58,7 +114,7 @@ __global__ __launch_bounds__(256) void k_tc(const TcArgs a) {
   const int64_t cb = (int64_t)c * a.chunk;
   const int64_t ce = min(cb + (int64_t)a.chunk, a.T);
   const W* __restrict__ At = (const W*)a.a_tiles;
-  const W* __restrict__ Bt = (const W*)a.b_tiles;
+  const W* __restrict__ Rt = (const W*)a.b_tiles;  // B's tiles in row form
   uint64_t total = 0;
 
   for (int32_t R0 = a.plan[c]; R0 < a.nbr;) {
@@ -79,12 +135,8 @@ __global__ __launch_bounds__(256) void k_tc(const TcArgs a) {
       const int64_t te = re < ce ? re : ce;
       for (int64_t tm = s; tm < te; ++tm) {  // mask tile (I, K)
         const int32_t K = a.a_colind[tm];
-        const uint64_t M = (lane < BS) ? (uint64_t)At[tm * BS + lane] : 0ull;
-        // Union of the mask's column bits: the k's any row needs.
-        uint64_t un = M;
-        for (int off = 1; off < kWave; off <<= 1) un |= __shfl_xor(un, off);
-        if (un == 0) continue;
-        // Intersect block row I of A (J ascending) with block column K of B.
+        // Intersect block row I of A (J ascending) with block column K of B;
+        // every lane that finds a match adds that (I,J,K) term itself.
         const int64_t la0 = rs, la1 = re;
         const int64_t lb0 = a.bt_ptr[K], lb1 = a.bt_ptr[K + 1];
         const bool a_short = (la1 - la0) <= (lb1 - lb0);
@@ -92,30 +144,49 @@ __global__ __launch_bounds__(256) void k_tc(const TcArgs a) {
         const int64_t l0 = a_short ? lb0 : la0, l1 = a_short ? lb1 : la1;
         const int32_t* skeys = a_short ? a.a_colind : a.bt_row;
         const int32_t* lkeys = a_short ? a.bt_row : a.a_colind;
+        // bs 32/64: one lane per tile row; mask row r stays in a register
+        const uint64_t mrow = (BS >= 32 && lane < (kWave / BS) * BS) ? (uint64_t)At[tm * BS + lane % BS] : 0ull;
         for (int64_t base = s0; base < s1; base += kWave) {
           const int64_t q = base + lane;
-          int64_t ta = -1, tb = -1;
+          int64_t tA = -1, tB = -1;
           if (q < s1) {
             const int32_t J = skeys[q];
             const int64_t pos = lower_bound_i32(lkeys, l0, l1, J);
             if (pos < l1 && lkeys[pos] == J) {
-              ta = a_short ? q : pos;
-              tb = a_short ? a.bt_tile[pos] : a.bt_tile[q];
+              tA = a_short ? q : pos;
+              tB = a_short ? a.bt_tile[pos] : a.bt_tile[q];
             }
           }
-          uint64_t hits = __ballot(ta >= 0);
-          while (hits) {
-            const int h = __builtin_ctzll(hits);
-            hits &= hits - 1;
-            const int64_t tA = __shfl(ta, h), tB = __shfl(tb, h);
-            const uint64_t X = (lane < BS) ? (uint64_t)At[tA * BS + lane] : 0ull;
-            const uint64_t Y = (lane < BS) ? (uint64_t)Bt[tB * BS + lane] : 0ull;
-            uint64_t u = un;
-            while (u) {
-              const int p = __builtin_ctzll(u);  // mask bit p <-> column k = BS-1-p
-              u &= u - 1;
-              const uint64_t yk = __shfl(Y, BS - 1 - p);
-              if ((M >> p) & 1ull) total += (uint64_t)popcw(X & yk);
+          if constexpr (BS <= 16) {
+            // each lane adds its own match
+            if (tA >= 0) total += tc_hit<BS>(At + tA * BS, At + tm * BS, Rt + tB * BS);
+          } else {
+            // 64/BS matches at a time, one per group of BS lanes (lane = row)
+            constexpr int G = kWave / BS;
+            const int grp = lane / BS, r = lane % BS;
+            uint64_t hits = __ballot(tA >= 0);
+            while (hits) {
+              int hsel = -1;
+#pragma unroll
+              for (int g2 = 0; g2 < G; ++g2) {
+                if (hits) {
+                  const int h = __builtin_ctzll(hits);
+                  hits &= hits - 1;
+                  if (g2 == grp) hsel = h;
+                }
+              }
+              const int src = hsel >= 0 ? hsel : 0;
+              const int64_t hA = __shfl(tA, src), hB = __shfl(tB, src);
+              if (hsel >= 0) {
+                uint64_t wa = (uint64_t)At[hA * BS + r];
+                if (mrow) {
+                  while (wa) {
+                    const int p = __builtin_ctzll(wa);
+                    wa &= wa - 1;
+                    total += (uint64_t)popcw((W)(mrow & (uint64_t)Rt[hB * BS + (BS - 1 - p)]));
+                  }
+                }
+              }
             }
           }
         }
@@ -228,6 +299,21 @@ int gbo_bsrbmm_tc(const gbo_bsr* L, const gbo_bsr* LT, uint64_t* d_count, gbo_st
   int32_t *bt_ptr = nullptr, *bt_row = nullptr, *bt_tile = nullptr;
   rc = block_transpose(LT, &bt_ptr, &bt_row, &bt_tile, st);
   if (rc) return rc;
+  // B's tiles in row form (same positions), for one popcount per set bit of A
+  void* rows = nullptr;
+  const size_t tbytes = (size_t)LT->nblocks * LT->bs * LT->bs / 8;
+  GBO_HIP_CHECK(hipMallocAsync(&rows, std::max<size_t>(tbytes, 8), st));
+  {
+    const int64_t waves = (LT->nblocks * LT->bs + kWave - 1) / kWave;
+    const unsigned tb = (unsigned)((waves + 3) / 4);
+    switch (LT->bs) {
+      case 8: k_tile_rows<8><<<tb, 256, 0, st>>>((const uint8_t*)LT->tiles, LT->nblocks, (uint8_t*)rows); break;
+      case 16: k_tile_rows<16><<<tb, 256, 0, st>>>((const uint16_t*)LT->tiles, LT->nblocks, (uint16_t*)rows); break;
+      case 32: k_tile_rows<32><<<tb, 256, 0, st>>>((const uint32_t*)LT->tiles, LT->nblocks, (uint32_t*)rows); break;
+      case 64: k_tile_rows<64><<<tb, 256, 0, st>>>((const uint64_t*)LT->tiles, LT->nblocks, (uint64_t*)rows); break;
+    }
+    GBO_LAUNCH_CHECK();
+  }
   // TC work per mask tile is far larger than a bsrbmv tile: small chunks.
   gbo_bsr tmp = *L;
   tmp.plan = nullptr;
@@ -246,7 +332,7 @@ int gbo_bsrbmm_tc(const gbo_bsr* L, const gbo_bsr* LT, uint64_t* d_count, gbo_st
     a.bt_ptr = bt_ptr;
     a.bt_row = bt_row;
     a.bt_tile = bt_tile;
-    a.b_tiles = LT->tiles;
+    a.b_tiles = rows;
     a.count = (unsigned long long*)d_count;
     const unsigned blocks = (unsigned)(((int64_t)a.nchunks + 3) / 4);
     switch (L->bs) {
@@ -262,6 +348,7 @@ int gbo_bsrbmm_tc(const gbo_bsr* L, const gbo_bsr* LT, uint64_t* d_count, gbo_st
   (void)hipFreeAsync(bt_ptr, st);
   (void)hipFreeAsync(bt_row, st);
   (void)hipFreeAsync(bt_tile, st);
+  (void)hipFreeAsync(rows, st);
   return rc;
   GBO_TRY_END
 }
