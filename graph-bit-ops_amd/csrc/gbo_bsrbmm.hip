@@ -5,10 +5,10 @@
 //   by columns (word k of B tile (J,K) = column k of that tile).
 // The reference walks, per mask tile (I,K), every tile (I,J) of block row I
 // and binary-searches B's block row J for K -- O(deg^2 log) per block row.
-// Here the block-column structure of B is transposed once (a stable counting
-// sort on the host, like the reference's csr2csc) so that the J's that
-// contribute to mask tile (I,K) are the intersection of two sorted lists:
-// block row I of A and block column K of B.  Each wave takes a fixed chunk
+// Here the block-column structure of B is transposed once (a stable radix
+// sort of the tiles by block column on the device: csr2csc on blocks) so
+// that the J's that contribute to mask tile (I,K) are the intersection of
+// two sorted lists: block row I of A and block column K of B.  Each wave takes a fixed chunk
 // of mask tiles; its 64 lanes search the longer list for 64 entries of the
 // shorter one at a time, and each lane that finds a match J adds that
 // term itself: with B's tiles turned into row form once per call,
@@ -20,6 +20,8 @@
 // two linear passes over the tiles instead of the reference's tile-pair loop.
 #include <algorithm>
 #include <vector>
+
+#include <rocprim/device/device_radix_sort.hpp>
 
 #include "gbo_device.hpp"
 #include "gbo_ops.hpp"
@@ -245,38 +247,68 @@ __global__ __launch_bounds__(256) void k_dot_u64(const unsigned long long* __res
   if (lane_id() == 0 && s) atomicAdd(out, (unsigned long long)s);
 }
 
+__global__ __launch_bounds__(256) void k_iota_i32(int32_t* __restrict__ v, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) v[i] = (int32_t)i;
+}
+
+// ptr[c] = first position of sorted keys >= c, for c in [0, nbc].
+__global__ __launch_bounds__(256) void k_col_bounds(const int32_t* __restrict__ keys, int64_t T, int32_t nbc,
+                                                    int32_t* __restrict__ ptr) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c > nbc) return;
+  ptr[c] = (int32_t)lower_bound_i32(keys, 0, T, (int32_t)c);
+}
+
+// row[i] = block row of tile tile[i]: the last J with rowptr[J] <= tile.
+__global__ __launch_bounds__(256) void k_tile_block_row(const int32_t* __restrict__ rowptr, int32_t nbr,
+                                                        const int32_t* __restrict__ tile, int64_t T,
+                                                        int32_t* __restrict__ row) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= T) return;
+  const int32_t t = tile[i];
+  int32_t lo = 0, hi = nbr;  // first J with rowptr[J + 1] > t
+  while (lo < hi) {
+    const int32_t mid = lo + (hi - lo) / 2;
+    if (rowptr[mid + 1] <= t) lo = mid + 1;
+    else hi = mid;
+  }
+  row[i] = lo;
+}
+
 }  // namespace gbo
 
 using namespace gbo;
 
-// Host-side stable transpose of a block structure (csr2csc on blocks).
+// Block-CSC of a block structure on the device (csr2csc on blocks): a stable
+// radix sort of the tile indices by block column.  The tiles are stored in
+// block-row order, so every column's list comes out with ascending rows.
 static int block_transpose(const gbo_bsr* B, int32_t** d_ptr, int32_t** d_row, int32_t** d_tile, hipStream_t st) {
   const int32_t nbr = B->nblockrows, nbc = B->nblockcols;
   const int64_t T = B->nblocks;
-  std::vector<int32_t> rp((size_t)nbr + 1), ci((size_t)T);
-  GBO_HIP_CHECK(hipMemcpyAsync(rp.data(), B->rowptr, sizeof(int32_t) * (nbr + 1), hipMemcpyDeviceToHost, st));
-  if (T) GBO_HIP_CHECK(hipMemcpyAsync(ci.data(), B->colind, sizeof(int32_t) * T, hipMemcpyDeviceToHost, st));
-  GBO_HIP_CHECK(hipStreamSynchronize(st));
-  std::vector<int32_t> cp((size_t)nbc + 1, 0), row((size_t)std::max<int64_t>(T, 1)), tile((size_t)std::max<int64_t>(T, 1));
-  for (int64_t t = 0; t < T; ++t) {
-    if (ci[t] < 0 || ci[t] >= nbc) return set_error(GBO_EINVAL, "block column index out of range");
-    cp[ci[t] + 1]++;
-  }
-  for (int32_t k = 0; k < nbc; ++k) cp[k + 1] += cp[k];
-  std::vector<int32_t> cur(cp.begin(), cp.end() - 1);
-  for (int32_t J = 0; J < nbr; ++J)
-    for (int32_t t = rp[J]; t < rp[J + 1]; ++t) {
-      const int32_t o = cur[ci[t]]++;
-      row[o] = J;
-      tile[o] = t;
-    }
+  *d_ptr = *d_row = *d_tile = nullptr;
+  int32_t *keys = nullptr, *idx = nullptr;
+  void* tmp = nullptr;
+  size_t tmp_bytes = 0;
+  unsigned bits = 1;
+  while (bits < 31 && (1ll << bits) <= nbc) ++bits;
   GBO_HIP_CHECK(hipMallocAsync((void**)d_ptr, sizeof(int32_t) * ((size_t)nbc + 1), st));
-  GBO_HIP_CHECK(hipMallocAsync((void**)d_row, sizeof(int32_t) * row.size(), st));
-  GBO_HIP_CHECK(hipMallocAsync((void**)d_tile, sizeof(int32_t) * tile.size(), st));
-  GBO_HIP_CHECK(hipMemcpyAsync(*d_ptr, cp.data(), sizeof(int32_t) * cp.size(), hipMemcpyHostToDevice, st));
-  GBO_HIP_CHECK(hipMemcpyAsync(*d_row, row.data(), sizeof(int32_t) * row.size(), hipMemcpyHostToDevice, st));
-  GBO_HIP_CHECK(hipMemcpyAsync(*d_tile, tile.data(), sizeof(int32_t) * tile.size(), hipMemcpyHostToDevice, st));
-  GBO_HIP_CHECK(hipStreamSynchronize(st));  // host vectors die at return
+  GBO_HIP_CHECK(hipMallocAsync((void**)d_row, sizeof(int32_t) * std::max<int64_t>(T, 1), st));
+  GBO_HIP_CHECK(hipMallocAsync((void**)d_tile, sizeof(int32_t) * std::max<int64_t>(T, 1), st));
+  GBO_HIP_CHECK(hipMallocAsync((void**)&keys, sizeof(int32_t) * std::max<int64_t>(T, 1), st));
+  GBO_HIP_CHECK(hipMallocAsync((void**)&idx, sizeof(int32_t) * std::max<int64_t>(T, 1), st));
+  k_iota_i32<<<(unsigned)((T + 255) / 256), 256, 0, st>>>(idx, T);
+  GBO_LAUNCH_CHECK();
+  GBO_HIP_CHECK(rocprim::radix_sort_pairs(nullptr, tmp_bytes, B->colind, keys, idx, *d_tile, (size_t)T, 0, bits, st));
+  GBO_HIP_CHECK(hipMallocAsync(&tmp, std::max<size_t>(tmp_bytes, 1), st));
+  GBO_HIP_CHECK(rocprim::radix_sort_pairs(tmp, tmp_bytes, B->colind, keys, idx, *d_tile, (size_t)T, 0, bits, st));
+  k_col_bounds<<<(unsigned)(((int64_t)nbc + 1 + 255) / 256), 256, 0, st>>>(keys, T, nbc, *d_ptr);
+  GBO_LAUNCH_CHECK();
+  k_tile_block_row<<<(unsigned)((T + 255) / 256), 256, 0, st>>>(B->rowptr, nbr, *d_tile, T, *d_row);
+  GBO_LAUNCH_CHECK();
+  (void)hipFreeAsync(tmp, st);
+  (void)hipFreeAsync(keys, st);
+  (void)hipFreeAsync(idx, st);
   return GBO_OK;
 }
 
