@@ -62,9 +62,69 @@ __global__ __launch_bounds__(256) void k_bfs_update(const word_t<BS>* __restrict
   }
 }
 
+// ---- push steps (direction-optimizing BFS) over A, the out-edge matrix.
+// Frontier block rows are cut into segments of at most kPushSeg tiles so that
+// a hub row is spread over many wavefronts.
+constexpr int32_t kPushSeg = 4096;
+
 template <int BS>
-static int bfs_t(const gbo_bsr* A, int32_t source, int32_t* d_level, int64_t* sizes, int32_t max_levels,
-                 int32_t* nlevels, hipStream_t st) {
+__global__ __launch_bounds__(256) void k_push_segs(const word_t<BS>* __restrict__ f, const int32_t* __restrict__ rowptr,
+                                                   int32_t nbr, int32_t* __restrict__ segcnt) {
+  const int64_t J = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (J > nbr) return;
+  segcnt[J] = (J < nbr && f[J] != 0) ? (rowptr[J + 1] - rowptr[J] + kPushSeg - 1) / kPushSeg : 0;
+}
+
+// nx[J'] |= the unvisited out-neighbours of the frontier rows of every tile
+// (J, J') of a frontier block row J; segments are taken grid-stride.
+template <int BS>
+__global__ __launch_bounds__(256) void k_push(const word_t<BS>* __restrict__ f, const word_t<BS>* __restrict__ vis,
+                                              word_t<BS>* __restrict__ nx, const int32_t* __restrict__ rowptr,
+                                              const int32_t* __restrict__ colind, const word_t<BS>* __restrict__ tiles,
+                                              const int32_t* __restrict__ segoff, int32_t nbr) {
+  const int lane = lane_id();
+  const int64_t nw = (int64_t)gridDim.x * (blockDim.x / kWave);
+  const int64_t S = segoff[nbr];
+  for (int64_t sg = wave_index(); sg < S; sg += nw) {
+    int32_t lo = 0, hi = nbr;  // block row: last J with segoff[J] <= sg
+    while (lo < hi) {
+      const int32_t mid = lo + (hi - lo + 1) / 2;
+      if (segoff[mid] <= sg) lo = mid;
+      else hi = mid - 1;
+    }
+    const int32_t J = lo;
+    const int64_t t0 = (int64_t)rowptr[J] + (sg - segoff[J]) * kPushSeg;
+    const int64_t t1 = min(t0 + kPushSeg, (int64_t)rowptr[J + 1]);
+    const uint64_t fw = (uint64_t)f[J];
+    for (int64_t t = t0 + lane; t < t1; t += kWave) {
+      uint64_t nb = 0;  // out-neighbours of the frontier rows of this tile
+      uint64_t rows = fw;
+      while (rows) {
+        const int p = __builtin_ctzll(rows);
+        rows &= rows - 1;
+        nb |= (uint64_t)tiles[t * BS + (BS - 1 - p)];
+      }
+      const int32_t J2 = colind[t];
+      nb &= ~(uint64_t)vis[J2] & low_mask<BS>();
+      if (nb) atomic_or_word<BS>(nx, J2, nb);
+    }
+  }
+}
+
+template <int BS>
+__global__ __launch_bounds__(256) void k_popc_words(const word_t<BS>* __restrict__ w, int32_t nbr,
+                                                    unsigned long long* __restrict__ cnt) {
+  const int64_t J = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t c = J < nbr ? (uint64_t)popcw(w[J]) : 0ull;
+  for (int off = 1; off < kWave; off <<= 1) c += __shfl_xor(c, off);
+  if (lane_id() == 0 && c) atomicAdd(cnt, (unsigned long long)c);
+}
+
+// Pull BFS over AT; with A (the out-edge matrix, AT itself for an undirected
+// graph) levels whose frontier holds at most n/64 vertices are pushed.
+template <int BS>
+static int bfs_t(const gbo_bsr* A, const gbo_bsr* Aout, int32_t source, int32_t* d_level, int64_t* sizes,
+                 int32_t max_levels, int32_t* nlevels, hipStream_t st) {
   using W = word_t<BS>;
   const int32_t nbr = A->nblockrows, n = A->nrows;
   W *f = nullptr, *nx = nullptr, *vis = nullptr;
@@ -73,6 +133,8 @@ static int bfs_t(const gbo_bsr* A, int32_t source, int32_t* d_level, int64_t* si
   nx = f + std::max(nbr, 1);
   vis = nx + std::max(nbr, 1);
   GBO_HIP_CHECK(hipMallocAsync((void**)&d_cnt, sizeof(int64_t), st));
+  int32_t* segoff = nullptr;
+  if (Aout) GBO_HIP_CHECK(hipMallocAsync((void**)&segoff, sizeof(int32_t) * ((size_t)nbr + 1), st));
   PlanView pv;
   int rc = acquire_plan(A, &pv, st);
   if (rc) return rc;
@@ -93,19 +155,34 @@ static int bfs_t(const gbo_bsr* A, int32_t source, int32_t* d_level, int64_t* si
   a.variant = pv.variant;
   a.count = d_cnt;
   int32_t lvl = 0;
+  int64_t fsize = 1;  // vertices in the current frontier
   if (sizes && max_levels > 0) sizes[0] = 1;
   while (true) {
     GBO_HIP_CHECK(hipMemsetAsync(d_cnt, 0, sizeof(int64_t), st));
-    a.xbits = f;
-    a.visited = vis;
-    a.out = nx;
-    rc = spmv_launch(BS, OP_BFS, a, st);
-    if (rc) break;
+    if (Aout && fsize <= std::max<int64_t>(n / 64, 1)) {
+      // push: only the frontier rows' tiles of A are read
+      GBO_HIP_CHECK(hipMemsetAsync(nx, 0, sizeof(W) * (size_t)std::max(nbr, 1), st));
+      k_push_segs<BS><<<(unsigned)(((int64_t)nbr + 1 + 255) / 256), 256, 0, st>>>(f, Aout->rowptr, nbr, segoff);
+      GBO_LAUNCH_CHECK();
+      rc = scan_exclusive<int32_t, int32_t>(segoff, segoff, (int64_t)nbr + 1, st);
+      if (rc) break;
+      k_push<BS><<<2048, 256, 0, st>>>(f, vis, nx, Aout->rowptr, Aout->colind, (const W*)Aout->tiles, segoff, nbr);
+      GBO_LAUNCH_CHECK();
+      k_popc_words<BS><<<(unsigned)((nbr + 255) / 256), 256, 0, st>>>(nx, nbr, (unsigned long long*)d_cnt);
+      GBO_LAUNCH_CHECK();
+    } else {
+      a.xbits = f;
+      a.visited = vis;
+      a.out = nx;
+      rc = spmv_launch(BS, OP_BFS, a, st);
+      if (rc) break;
+    }
     int64_t cnt = 0;
     GBO_HIP_CHECK(hipMemcpyAsync(&cnt, d_cnt, sizeof(int64_t), hipMemcpyDeviceToHost, st));
     GBO_HIP_CHECK(hipStreamSynchronize(st));
     if (cnt == 0) break;
     ++lvl;
+    fsize = cnt;
     if (sizes && lvl < max_levels) sizes[lvl] = cnt;
     k_bfs_update<BS><<<(unsigned)((nbr + 255) / 256), 256, 0, st>>>(nx, vis, d_level, nbr, n, lvl);
     GBO_LAUNCH_CHECK();
@@ -116,6 +193,7 @@ static int bfs_t(const gbo_bsr* A, int32_t source, int32_t* d_level, int64_t* si
   W* base = std::min(std::min(f, nx), vis);
   (void)hipFreeAsync(base, st);
   (void)hipFreeAsync(d_cnt, st);
+  if (segoff) (void)hipFreeAsync(segoff, st);
   GBO_HIP_CHECK(hipStreamSynchronize(st));
   return rc;
 }
@@ -277,12 +355,32 @@ int gbo_bfs(const gbo_bsr* AT, int32_t source, int32_t* d_level, int64_t* h_leve
   if (source < 0 || source >= AT->nrows) return set_error(GBO_EINVAL, "gbo_bfs: source %d out of range", source);
   if (!d_level) return set_error(GBO_EINVAL, "gbo_bfs: null level");
   switch (AT->bs) {
-    case 8: return bfs_t<8>(AT, source, d_level, h_level_sizes, max_levels, h_nlevels, stream);
-    case 16: return bfs_t<16>(AT, source, d_level, h_level_sizes, max_levels, h_nlevels, stream);
-    case 32: return bfs_t<32>(AT, source, d_level, h_level_sizes, max_levels, h_nlevels, stream);
-    case 64: return bfs_t<64>(AT, source, d_level, h_level_sizes, max_levels, h_nlevels, stream);
+    case 8: return bfs_t<8>(AT, nullptr, source, d_level, h_level_sizes, max_levels, h_nlevels, stream);
+    case 16: return bfs_t<16>(AT, nullptr, source, d_level, h_level_sizes, max_levels, h_nlevels, stream);
+    case 32: return bfs_t<32>(AT, nullptr, source, d_level, h_level_sizes, max_levels, h_nlevels, stream);
+    case 64: return bfs_t<64>(AT, nullptr, source, d_level, h_level_sizes, max_levels, h_nlevels, stream);
   }
   return set_error(GBO_EUNSUPPORTED, "gbo_bfs: block size");
+}
+
+int gbo_bfs_do(const gbo_bsr* AT, const gbo_bsr* A, int32_t source, int32_t* d_level, int64_t* h_level_sizes,
+               int32_t max_levels, int32_t* h_nlevels, gbo_stream_t stream) {
+  int rc = check_square(AT, "gbo_bfs_do");
+  if (rc) return rc;
+  if (A) {
+    rc = check_square(A, "gbo_bfs_do(A)");
+    if (rc) return rc;
+    if (A->bs != AT->bs || A->nrows != AT->nrows) return set_error(GBO_EINVAL, "gbo_bfs_do: A and AT differ in shape");
+  }
+  if (source < 0 || source >= AT->nrows) return set_error(GBO_EINVAL, "gbo_bfs_do: source %d out of range", source);
+  if (!d_level) return set_error(GBO_EINVAL, "gbo_bfs_do: null level");
+  switch (AT->bs) {
+    case 8: return bfs_t<8>(AT, A, source, d_level, h_level_sizes, max_levels, h_nlevels, stream);
+    case 16: return bfs_t<16>(AT, A, source, d_level, h_level_sizes, max_levels, h_nlevels, stream);
+    case 32: return bfs_t<32>(AT, A, source, d_level, h_level_sizes, max_levels, h_nlevels, stream);
+    case 64: return bfs_t<64>(AT, A, source, d_level, h_level_sizes, max_levels, h_nlevels, stream);
+  }
+  return set_error(GBO_EUNSUPPORTED, "gbo_bfs_do: block size");
 }
 
 int gbo_sssp(const gbo_bsr* AT, const uint32_t* d_w, const int64_t* d_woff, int32_t source, uint32_t* d_dist,

@@ -125,6 +125,7 @@ def _declare(L):
         "gbo_bsrbmm_tc": ([POINTER(BsrStruct), POINTER(BsrStruct), P, P], c_int),
         "gbo_bsrbmm_sum": ([POINTER(BsrStruct), POINTER(BsrStruct), P, P], c_int),
         "gbo_bfs": ([POINTER(BsrStruct), c_int32, P, P, c_int32, i32p, P], c_int),
+        "gbo_bfs_do": ([POINTER(BsrStruct), POINTER(BsrStruct), c_int32, P, P, c_int32, i32p, P], c_int),
         "gbo_sssp": ([POINTER(BsrStruct), P, P, c_int32, P, i32p, P], c_int),
         "gbo_pagerank": ([POINTER(BsrStruct), P, c_float, c_double, c_int32, P, i32p, POINTER(c_double), P], c_int),
         "gbo_cc": ([POINTER(BsrStruct), P, i32p, P], c_int),
@@ -515,12 +516,17 @@ def bsrbmm_sum(A_row: BsrMatrix, B_col: BsrMatrix) -> int:
 
 
 # ----------------------------------------------------------------- drivers
-def bfs(AT: BsrMatrix, source: int = 0, max_levels: int = 4096):
-    """(levels int32[n], per-level discovered counts)."""
+def bfs(AT: BsrMatrix, source: int = 0, max_levels: int = 4096, A: BsrMatrix | None = None):
+    """(levels int32[n], per-level discovered counts).  With A (the out-edge
+    matrix; AT itself for an undirected graph) small frontiers are pushed
+    (gbo_bfs_do); without it every level is pulled over AT (gbo_bfs)."""
     dl = DeviceBuffer(4 * max(AT.nrows, 1))
     sizes = np.zeros(max_levels, np.int64)
     nl = c_int32()
-    _check(lib().gbo_bfs(AT.ref, source, dl.ptr, _ptr(sizes), max_levels, byref(nl), None))
+    if A is None:
+        _check(lib().gbo_bfs(AT.ref, source, dl.ptr, _ptr(sizes), max_levels, byref(nl), None))
+    else:
+        _check(lib().gbo_bfs_do(AT.ref, A.ref, source, dl.ptr, _ptr(sizes), max_levels, byref(nl), None))
     return dl.to_numpy(np.int32, AT.nrows), sizes[: min(nl.value, max_levels)].copy()
 
 

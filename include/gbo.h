@@ -260,6 +260,14 @@ int gbo_bsrbmm_sum(const gbo_bsr* A_row, const gbo_bsr* B_col, uint64_t* d_sum,
 int gbo_bfs(const gbo_bsr* AT, int32_t source, int32_t* d_level,
             int64_t* h_level_sizes, int32_t max_levels, int32_t* h_nlevels,
             gbo_stream_t stream);
+/* Direction-optimizing BFS, same outputs as gbo_bfs.  A is the out-edge
+ * matrix (pass AT again for an undirected graph; NULL = pull only): a level
+ * whose frontier holds at most n/64 vertices is pushed along the frontier
+ * rows of A (only their tiles are read), the others are pulled over AT.
+ * No reference counterpart (algorithm/bfs.cu always pulls). */
+int gbo_bfs_do(const gbo_bsr* AT, const gbo_bsr* A, int32_t source, int32_t* d_level,
+               int64_t* h_level_sizes, int32_t max_levels, int32_t* h_nlevels,
+               gbo_stream_t stream);
 
 /* SSSP (algorithm/sssp.cu:420-488) generalised to uint32 weights:
  * frontier Bellman-Ford on the min-plus semiring; dist = INF (0xFFFFFFFF)

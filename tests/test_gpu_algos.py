@@ -42,6 +42,38 @@ def test_bfs_levels(gbo, po, graph_cases, bs):
             assert np.array_equal(olv, want) and np.array_equal(osz, hist), (name, s)
 
 
+@pytest.mark.parametrize("bs", BS)
+def test_bfs_direction_optimizing(gbo, po, graph_cases, bs):
+    """gbo_bfs_do (push for frontiers of at most n/64 vertices, pull over AT
+    otherwise) == the oracle's queue BFS: undirected cases with A = AT, and
+    a directed graph with A and AT packed separately."""
+    for name, g in graph_cases:
+        A = gbo.BsrMatrix.from_csr(g, bs)
+        for s in sources(g):
+            lv, sizes = gbo.bfs(A, s, A=A)
+            want = po.bfs(g.nrows, g.rowptr, g.colind, s)
+            assert np.array_equal(lv, want), (name, s)
+            assert np.array_equal(sizes, np.bincount(want[want >= 0])), (name, s)
+    rng = np.random.default_rng(bs)
+    n = 3000
+    src = rng.integers(0, n, 12000)
+    dst = rng.integers(0, n, 12000)
+    keep = src != dst
+    order = np.lexsort((dst[keep], src[keep]))
+    e = np.unique(np.stack([src[keep][order], dst[keep][order]], 1), axis=0)
+    rowptr = np.zeros(n + 1, np.int32)
+    rowptr[1:] = np.cumsum(np.bincount(e[:, 0], minlength=n))
+    g = gbo.Csr(n, n, rowptr, e[:, 1].astype(np.int32))
+    Ad = gbo.BsrMatrix.from_csr(g, bs)
+    ATd = gbo.BsrMatrix.from_csr(g.transpose(), bs)
+    for s in (0, 17, 2999):
+        want = po.bfs(n, g.rowptr, g.colind, s)
+        lv, _ = gbo.bfs(ATd, s, A=Ad)
+        assert np.array_equal(lv, want), ("directed", s)
+        lv2, _ = gbo.bfs(ATd, s)
+        assert np.array_equal(lv2, want), ("directed pull", s)
+
+
 def test_bfs_closed_forms(gbo):
     from graphs import path_graph, star_graph
     rp, ci = path_graph(1000)
