@@ -155,14 +155,17 @@ __global__ __launch_bounds__(256) void k_spmv(const SpmvArgs a) {
           int32_t cc[4];
           uint64_t wd[4];
           int rank[4];
+          int64_t wo[4];  // first weight of each tile
 #pragma unroll
           for (int u = 0; u < 4; ++u) {
             const int64_t tu = t + u * G;
             cc[u] = 0;
             wd[u] = 0;
+            wo[u] = 0;
             if (tu < te) {
               cc[u] = colind[tu];
               wd[u] = (uint64_t)tiles[tu * BS + r];
+              if (a.w) wo[u] = a.woff[tu];
               // inactive source block column: the tile is skipped (group-uniform)
               if (a.active && !((a.active[cc[u] >> 6] >> (cc[u] & 63)) & 1ull)) wd[u] = 0;
             }
@@ -179,17 +182,23 @@ __global__ __launch_bounds__(256) void k_spmv(const SpmvArgs a) {
             rank[u] = incl - 1;  // set bits are taken from the last column back
           }
           while (wd[0] | wd[1] | wd[2] | wd[3]) {
+            uint32_t dv[4], wt[4];  // the gathers of a pass are issued before any is used
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
+              dv[u] = kInfU32;
+              wt[u] = 0u;
               if (wd[u]) {
                 const int p = __builtin_ctzll(wd[u]);
-                const uint32_t dv = dist[(int64_t)cc[u] * BS + (BS - 1 - p)];
-                const uint32_t wt = a.w ? a.w[a.woff[t + u * G] + rank[u]] : 1u;
-                const uint32_t cand = (dv > kInfU32 - wt) ? kInfU32 : dv + wt;
-                acc = cand < acc ? cand : acc;
+                dv[u] = dist[(int64_t)cc[u] * BS + (BS - 1 - p)];
+                wt[u] = a.w ? a.w[wo[u] + rank[u]] : 1u;
                 wd[u] &= wd[u] - 1;
                 --rank[u];
               }
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+              const uint32_t cand = (dv[u] > kInfU32 - wt[u]) ? kInfU32 : dv[u] + wt[u];
+              acc = cand < acc ? cand : acc;
             }
           }
         }
@@ -260,15 +269,18 @@ __global__ __launch_bounds__(256) void k_spmv(const SpmvArgs a) {
             wd[u] = tu < te ? (uint64_t)tiles[tu * BS + r] : 0ull;
           }
           while (wd[0] | wd[1] | wd[2] | wd[3]) {
+            int32_t v[4];  // the 4 gathers of a pass are issued before any is used
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
+              v[u] = INT32_MAX;
               if (wd[u]) {
                 const int p = __builtin_ctzll(wd[u]);
-                const int32_t v = x[(int64_t)cc[u] * BS + (BS - 1 - p)];
-                acc = v < acc ? v : acc;
+                v[u] = x[(int64_t)cc[u] * BS + (BS - 1 - p)];
                 wd[u] &= wd[u] - 1;
               }
             }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) acc = v[u] < acc ? v[u] : acc;
           }
         }
         for (int off = BS; off < kWave; off <<= 1) {
