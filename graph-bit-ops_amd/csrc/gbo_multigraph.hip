@@ -5,19 +5,35 @@
 //
 // The graphs are packed side by side as one block-diagonal bit-BSR: graph g
 // occupies vertices [vbase_g, vbase_g + n_g) with vbase_g a multiple of bs,
-// so every tile belongs to exactly one graph.  A persistent kernel is
-// launched with a few workgroups per CU; each workgroup takes the next graph
-// from an atomic queue (largest first) and runs that graph's BFS (pull,
-// frontier bit words) and FastSV CC to convergence with workgroup barriers
-// only -- graphs terminate independently, without host synchronisation.
-// The launch (queue reset kernel + persistent kernel) is captured once in a
-// hipGraph and replayed.
+// so every tile (and every block row) belongs to exactly one graph.  One
+// persistent kernel, sized so that all of its workgroups are resident at
+// once, sweeps the block rows of every graph together: BFS levels and FastSV
+// iterations advance in lockstep across the batch, separated by grid-wide
+// barriers, so a large graph's rows are spread over the whole GPU instead of
+// one workgroup.  Per-graph state keeps each graph's own schedule: a graph
+// whose frontier emptied (BFS) or that stopped changing (FastSV) skips its
+// rows from then on, exactly where a one-graph run would have stopped.  The
+// barrier wait is bounded: a wait past the limit sets an abort flag, every
+// workgroup leaves, and the host reports an error.  The launch (state reset
+// kernel + persistent kernel) is captured once in a hipGraph and replayed.
 #include <algorithm>
 #include <numeric>
 #include <vector>
 
 #include "gbo_device.hpp"
 #include "gbo_ops.hpp"
+
+namespace gbo {
+// Grid barrier state and the rotating "anything changed" counters.
+struct GridSync {
+  unsigned int arrive;
+  unsigned int gen;
+  unsigned int abort;
+  unsigned int cnt[3];
+  unsigned int join;     // tickets taken (| closed bit)
+  unsigned int members;  // member count, published at the close
+};
+}  // namespace gbo
 
 struct gbo_batch {
   int bs = 0;
@@ -26,21 +42,20 @@ struct gbo_batch {
   int64_t nunion = 0;          // padded vertex count
   std::vector<int64_t> out_off;  // per-graph offset in the outputs (unpadded), ngraphs+1
   // device descriptors
-  int32_t* d_blk0 = nullptr;   // first block row
-  int32_t* d_blk1 = nullptr;   // one past the last block row
+  int32_t* d_rowg = nullptr;   // graph of each block row
   int64_t* d_vbase = nullptr;  // first union vertex
   int32_t* d_n = nullptr;      // vertex count
   int64_t* d_out = nullptr;    // output offset
-  int32_t* d_order = nullptr;  // processing order (largest first)
   int32_t* d_src = nullptr;    // BFS sources (graph-local)
   // scratch over the union
-  void* d_f = nullptr;         // frontier bits (nblockrows words)
-  void* d_nx = nullptr;
+  void* d_f0 = nullptr;        // frontier bits, two levels (nblockrows words each)
+  void* d_f1 = nullptr;
   void* d_vis = nullptr;
   int32_t* d_lab = nullptr;    // FastSV parent (union ids)
   int32_t* d_gf = nullptr;
   int32_t* d_mngf = nullptr;
-  unsigned int* d_queue = nullptr;
+  int32_t* d_act = nullptr;    // per graph: BFS level, then FastSV iteration, of its last change (2 x ngraphs)
+  gbo::GridSync* d_sync = nullptr;
   // captured launch
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -54,174 +69,626 @@ struct BatchArgs {
   const int32_t* rowptr;
   const int32_t* colind;
   const void* tiles;
+  int64_t nblockrows;
   int32_t ngraphs;
-  const int32_t* blk0;
-  const int32_t* blk1;
+  const int32_t* rowg;
   const int64_t* vbase;
   const int32_t* n;
   const int64_t* out;
-  const int32_t* order;
   const int32_t* src;
-  void* f;
-  void* nx;
+  void* f0;
+  void* f1;
   void* vis;
   int32_t* lab;
   int32_t* gf;
   int32_t* mngf;
+  int32_t* bfs_act;  // last BFS level at which the graph found vertices
+  int32_t* cc_act;   // last FastSV iteration that changed the graph
   int32_t* level;
   int32_t* label;
-  unsigned int* queue;
+  GridSync* sync;
+  int64_t spin_ticks;  // barrier wait limit, wall-clock ticks
+  int64_t join_ticks;  // how long membership stays open, wall-clock ticks
 };
 
 constexpr int kBatchThreads = 512;
 constexpr int kBatchWaves = kBatchThreads / kWave;
+constexpr int kBatchSpinMs = 5000;       // barrier wait limit
+constexpr int kBatchJoinUs = 50;         // how long membership stays open
+// Block rows per wave task: bs 8/16 take a group of rows as one tile range;
+// bs 32/64 rows are long enough alone.  Concurrent tasks then cover a
+// window of rows (and so of graphs) small enough for their gathers to stay
+// in L2.
+template <int BS>
+constexpr int row_group() { return BS == 8 ? 16 : BS == 16 ? 8 : 1; }
+constexpr unsigned int kJoinClosed = 0x80000000u;
 
-__global__ void k_batch_reset(unsigned int* q) { *q = 0; }
+__global__ void k_batch_reset(GridSync* s) {
+  s->arrive = 0;
+  s->gen = 0;
+  s->abort = 0;
+  s->cnt[0] = s->cnt[1] = s->cnt[2] = 0;
+  s->join = 0;
+  s->members = 0;
+}
+
+__device__ __forceinline__ unsigned int load_agent(const unsigned int* p) {
+  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void set_abort(GridSync* s) {
+  __hip_atomic_store(&s->abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Membership.  A workgroup joins by taking a ticket; the holder of ticket 0
+// closes the membership once every workgroup of the grid has joined or after
+// join_ticks, and publishes the member count.  A workgroup that joins after
+// the close was not running alongside the members and leaves at once, so the
+// grid barrier only ever waits for workgroups that are resident.
+__device__ bool grid_join(GridSync* s, int64_t join_ticks, int64_t spin_ticks, int& part, int& nparts) {
+  __shared__ int s_part, s_np;
+  if (threadIdx.x == 0) {
+    int p = -1, np = 0;
+    const unsigned int t = __hip_atomic_fetch_add(&s->join, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    if (!(t & kJoinClosed)) {
+      p = (int)t;
+      if (t == 0) {
+        const int64_t t0 = wall_clock64();
+        while (load_agent(&s->join) < gridDim.x && wall_clock64() - t0 < join_ticks) __builtin_amdgcn_s_sleep(2);
+        const unsigned int n =
+            __hip_atomic_fetch_or(&s->join, kJoinClosed, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) & ~kJoinClosed;
+        __hip_atomic_store(&s->members, n, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        np = (int)n;
+      } else {
+        const int64_t t0 = wall_clock64();
+        while ((np = (int)load_agent(&s->members)) == 0) {
+          if (wall_clock64() - t0 > spin_ticks) {
+            set_abort(s);
+            p = -1;
+            break;
+          }
+          __builtin_amdgcn_s_sleep(2);
+        }
+      }
+    }
+    s_part = p;
+    s_np = np;
+  }
+  __syncthreads();
+  part = s_part;
+  nparts = s_np;
+  return part >= 0;
+}
+
+// Grid-wide barrier over the members, and counted phases: workgroups that
+// saw a change add 1 to cnt[ph % 3], the barrier completes, and everyone
+// reads the total.  Slot (ph+1) % 3 is cleared by member 0 during phase ph:
+// its last readers finished before the barrier that closed phase ph-1.  Every
+// wait is bounded; past the limit the abort flag is set and all members
+// leave.
+struct GridBarrier {
+  GridSync* s;
+  int64_t spin_ticks;
+  unsigned int nparts;
+  bool leader;
+  unsigned int gen = 0, ph = 0;
+
+  __device__ bool sync() {
+    __shared__ int s_ok;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      __threadfence();
+      const unsigned int prev = __hip_atomic_fetch_add(&s->arrive, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+      if (prev == nparts - 1) {
+        __hip_atomic_store(&s->arrive, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_fetch_add(&s->gen, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      } else {
+        const int64_t t0 = wall_clock64();
+        while (load_agent(&s->gen) == gen) {
+          if (load_agent(&s->abort)) break;
+          if (wall_clock64() - t0 > spin_ticks) {
+            set_abort(s);
+            break;
+          }
+          __builtin_amdgcn_s_sleep(2);
+        }
+      }
+      __threadfence();
+      s_ok = load_agent(&s->abort) == 0;
+    }
+    ++gen;
+    __syncthreads();
+    return s_ok != 0;
+  }
+
+  __device__ void open(int* s_any) {
+    if (threadIdx.x == 0) {
+      *s_any = 0;
+      if (leader) __hip_atomic_store(&s->cnt[(ph + 1) % 3], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+  }
+
+  // 1 if any member saw a change, 0 if none, -1 on abort
+  __device__ int close(const int* s_any) {
+    __shared__ int s_total;
+    __syncthreads();
+    if (threadIdx.x == 0 && *s_any)
+      __hip_atomic_fetch_add(&s->cnt[ph % 3], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (!sync()) return -1;
+    if (threadIdx.x == 0) s_total = load_agent(&s->cnt[ph % 3]) != 0;
+    ++ph;
+    __syncthreads();
+    return s_total;
+  }
+};
+
+// Bit k of the result: byte k of m is nonzero.
+__device__ __forceinline__ uint32_t nz_bytes(uint64_t m) {
+  const uint64_t t = (((m & 0x7f7f7f7f7f7f7f7full) + 0x7f7f7f7f7f7f7f7full) | m) & 0x8080808080808080ull;
+  return (uint32_t)(((t >> 7) * 0x0102040810204080ull) >> 56);
+}
+
+// Bit j of the result: 16-bit field j of m is nonzero.
+__device__ __forceinline__ uint32_t nz_shorts(uint64_t m) {
+  const uint64_t t = (((m & 0x7fff7fff7fff7fffull) + 0x7fff7fff7fff7fffull) | m) & 0x8000800080008000ull;
+  return (uint32_t)(((t >> 15) * 0x0001000200040008ull) >> 48);
+}
+
+__device__ __forceinline__ uint32_t wave_or(uint32_t v) {
+  for (int off = kWave / 2; off >= 1; off >>= 1) v |= __shfl_xor(v, off);
+  return v;
+}
+
+__device__ __forceinline__ int32_t wave_min(int32_t v) {
+  for (int off = kWave / 2; off >= 1; off >>= 1) {
+    const int32_t o = __shfl_xor(v, off);
+    v = o < v ? o : v;
+  }
+  return v;
+}
+
+__device__ __forceinline__ int64_t readlane64(int64_t v, int j) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, j);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)((uint64_t)v >> 32), j);
+  return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
+// Tile words of tile t as little-endian uint64s: bs 8 -> 1 word (byte r =
+// row r), bs 16 -> 4 words (word q holds rows 4q..4q+3 as 16-bit fields).
+template <int BS>
+__device__ __forceinline__ void tile_words(const void* tiles, int64_t t, uint64_t* w) {
+  if constexpr (BS == 8) {
+    w[0] = ((const uint64_t*)tiles)[t];
+  } else {
+    const uint4* p = (const uint4*)tiles + 2 * t;
+    const uint4 lo = p[0], hi = p[1];
+    w[0] = lo.x | ((uint64_t)lo.y << 32);
+    w[1] = lo.z | ((uint64_t)lo.w << 32);
+    w[2] = hi.x | ((uint64_t)hi.y << 32);
+    w[3] = hi.z | ((uint64_t)hi.w << 32);
+  }
+}
+
+// Pull-BFS over the tiles [rs, re) of one block row, bs 8/16: lane per tile.
+// Column indices of 64 tiles are one coalesced load, their frontier words
+// one gather, and a tile is read only if its frontier word is nonzero.
+// Returns (wave-uniform) bit r set iff row r has a frontier neighbour.
+template <int BS>
+__device__ __forceinline__ uint64_t bfs_row_lpt(const BatchArgs& a, const word_t<BS>* fk, int32_t rs, int32_t re,
+                                                int lane) {
+  constexpr int NS = (BS == 8) ? 4 : 2;  // tiles per lane per round
+  constexpr int NWD = BS / 4 > 1 ? BS / 4 : 1;  // words per tile: 1 (bs 8) or 4 (bs 16)
+  const int32_t* __restrict__ colind = a.colind;
+  uint32_t rows = 0;
+  for (int32_t t0 = rs; t0 < re; t0 += NS * kWave) {
+    int32_t c[NS];
+    uint32_t f[NS];
+#pragma unroll
+    for (int u = 0; u < NS; ++u) {
+      const int32_t t = t0 + u * kWave + lane;
+      c[u] = t < re ? colind[t] : -1;
+    }
+#pragma unroll
+    for (int u = 0; u < NS; ++u) f[u] = c[u] >= 0 ? (uint32_t)fk[c[u]] : 0u;
+#pragma unroll
+    for (int u = 0; u < NS; ++u) {
+      if (f[u]) {
+        uint64_t w[NWD];
+        tile_words<BS>(a.tiles, t0 + u * kWave + lane, w);
+        if constexpr (BS == 8) {
+          rows |= nz_bytes(w[0] & (0x0101010101010101ull * f[u]));
+        } else {
+          const uint64_t rep = 0x0001000100010001ull * f[u];
+#pragma unroll
+          for (int q = 0; q < 4; ++q) rows |= nz_shorts(w[q] & rep) << (4 * q);
+        }
+      }
+    }
+  }
+  return wave_or(rows);
+}
+
+// Pull-BFS over one block row, bs 32/64: lane per tile for the column
+// indices and frontier words, then the tiles with a nonzero frontier word
+// are read row per lane (64/bs tiles per step, eight tiles in flight).
+template <int BS>
+__device__ __forceinline__ uint64_t bfs_row_rl(const BatchArgs& a, const word_t<BS>* fk, int32_t rs, int32_t re,
+                                               int lane) {
+  using W = word_t<BS>;
+  constexpr int G = kWave / BS;
+  const int grp = lane / BS, r = lane % BS;
+  const int32_t* __restrict__ colind = a.colind;
+  const W* __restrict__ tiles = (const W*)a.tiles;
+  int hit = 0;
+  for (int32_t t0 = rs; t0 < re; t0 += kWave) {
+    const int32_t t = t0 + lane;
+    const W f = t < re ? fk[colind[t]] : (W)0;
+    uint64_t act = __ballot(f != 0);
+    while (act) {
+      constexpr int D = 8 / G;  // steps in flight: eight tiles per wave
+      int jj[D];
+#pragma unroll
+      for (int u = 0; u < D; ++u) {
+        int js = -1;
+#pragma unroll
+        for (int q = 0; q < G; ++q) {
+          const int j = act ? __builtin_ctzll(act) : -1;
+          act &= act - 1;
+          if (q == grp) js = j;
+        }
+        jj[u] = js;
+      }
+      W ww[D], ff[D];
+#pragma unroll
+      for (int u = 0; u < D; ++u) {
+        ww[u] = jj[u] >= 0 ? tiles[(int64_t)(t0 + jj[u]) * BS + r] : (W)0;
+        ff[u] = __shfl(f, jj[u] >= 0 ? jj[u] : 0);
+      }
+#pragma unroll
+      for (int u = 0; u < D; ++u) hit |= (W)(ww[u] & ff[u]) != 0;
+    }
+  }
+  for (int off = BS; off < kWave; off <<= 1) hit |= __shfl_xor(hit, off);
+  return __ballot(hit) & low_mask<BS>();
+}
+
+constexpr int kListCap = 8 * kWave;  // set-bit list entries per wave and pass
+
+// mngf[v] = min over the neighbours j of v of gf[j], for the vertices of the
+// block rows R0 .. R0+nrow-1 whose bit is set in amask, bs 8/16.  The rows'
+// tiles are contiguous and are taken lane per tile, 64*NS at a time, across
+// row boundaries.  A wave prefix sum of the tiles' popcounts places every set
+// bit (vertex, neighbour) in a wave-private LDS list, so the gathers of gf are
+// spread evenly over the lanes (eight per lane in flight) however the bits
+// fall in the tiles; each gathered value is folded into its vertex's
+// accumulator with an LDS atomic min.
+template <int BS, int RG>
+__device__ __forceinline__ void cc_group_lpt(const BatchArgs& a, int64_t R0, int nrow, int32_t l_rs, int32_t l_re,
+                                             uint64_t amask, int lane, uint32_t* s_idx, uint8_t* s_row,
+                                             int32_t* s_acc) {
+  constexpr int NS = (BS == 8) ? 4 : 2;
+  constexpr int NWD = BS / 4 > 1 ? BS / 4 : 1;
+  const int32_t* __restrict__ colind = a.colind;
+  for (int i = lane; i < RG * BS; i += kWave) s_acc[i] = INT32_MAX;
+  const int32_t gb = __builtin_amdgcn_readlane(l_rs, 0), ge = __builtin_amdgcn_readlane(l_re, nrow - 1);
+  int32_t rend[RG - 1];  // row ends (wave-uniform)
+#pragma unroll
+  for (int j = 0; j < RG - 1; ++j) rend[j] = j < nrow ? __builtin_amdgcn_readlane(l_re, j) : ge;
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  for (int32_t t0 = gb; t0 < ge; t0 += NS * kWave) {
+    uint32_t cb[NS];
+    int rr[NS];
+    uint64_t w[NS][NWD];
+    int cnt = 0;
+#pragma unroll
+    for (int u = 0; u < NS; ++u) {
+      const int32_t t = t0 + u * kWave + lane;
+      cb[u] = 0;
+      rr[u] = 0;
+#pragma unroll
+      for (int q = 0; q < NWD; ++q) w[u][q] = 0;
+      if (t < ge) {
+        cb[u] = (uint32_t)colind[t] * BS;
+        tile_words<BS>(a.tiles, t, w[u]);
+        int r = 0;
+#pragma unroll
+        for (int j = 0; j < RG - 1; ++j) r += t >= rend[j];
+        rr[u] = r * BS;
+        if (!((amask >> r) & 1)) {
+#pragma unroll
+          for (int q = 0; q < NWD; ++q) w[u][q] = 0;
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < NWD; ++q) cnt += __popcll(w[u][q]);
+    }
+    int incl = cnt;  // inclusive prefix sum of the popcounts over the lanes
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+      const int o = __shfl_up(incl, off);
+      if (lane >= off) incl += o;
+    }
+    const int excl = incl - cnt;
+    const int tot = __builtin_amdgcn_readlane(incl, kWave - 1);
+    for (int base = 0; base < tot; base += kListCap) {
+      int idx = excl;
+#pragma unroll
+      for (int u = 0; u < NS; ++u) {
+#pragma unroll
+        for (int q = 0; q < NWD; ++q) {
+          uint64_t ww = w[u][q];
+          while (ww) {
+            const int b = __builtin_ctzll(ww);
+            ww &= ww - 1;
+            if (idx >= base && idx < base + kListCap) {
+              // bs 8: byte b/8 = row, bit 7 - col;  bs 16: word q holds rows 4q..4q+3
+              const int row = (BS == 8) ? (b >> 3) : (4 * q + (b >> 4));
+              const int col = (BS == 8) ? (7 - (b & 7)) : (15 - (b & 15));
+              s_idx[idx - base] = cb[u] + col;
+              s_row[idx - base] = (uint8_t)(rr[u] + row);
+            }
+            ++idx;
+          }
+        }
+      }
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      const int nl = tot - base < kListCap ? tot - base : kListCap;
+      int32_t x[kListCap / kWave];
+      int rw[kListCap / kWave];
+#pragma unroll
+      for (int k = 0; k < kListCap / kWave; ++k) {
+        const int e = k * kWave + lane;
+        x[k] = INT32_MAX;
+        rw[k] = 0;
+        if (e < nl) {
+          rw[k] = s_row[e];
+          x[k] = a.gf[s_idx[e]];
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < kListCap / kWave; ++k)
+        if (k * kWave + lane < nl) atomicMin(&s_acc[rw[k]], x[k]);
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    }
+  }
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  for (int i = lane; i < nrow * BS; i += kWave)
+    if ((amask >> (i / BS)) & 1) a.mngf[R0 * BS + i] = s_acc[i];
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+}
+
+// mngf for one block row, bs 32/64: row per lane, four tiles per group per
+// round, one set bit of each per pass (the pass's gathers issued together).
+template <int BS>
+__device__ __forceinline__ void cc_row_rl(const BatchArgs& a, int64_t R, int32_t rs, int32_t re, int lane) {
+  using W = word_t<BS>;
+  constexpr int G = kWave / BS;
+  const int grp = lane / BS, r = lane % BS;
+  const int32_t* __restrict__ colind = a.colind;
+  const W* __restrict__ tiles = (const W*)a.tiles;
+  int32_t acc = INT32_MAX;
+  for (int64_t t = rs + grp; t < re; t += 4 * G) {
+    int32_t cc[4];
+    uint64_t wd[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int64_t tu = t + u * G;
+      cc[u] = tu < re ? colind[tu] : 0;
+      wd[u] = tu < re ? (uint64_t)tiles[tu * BS + r] : 0ull;
+    }
+    while (wd[0] | wd[1] | wd[2] | wd[3]) {
+      int32_t x[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        x[u] = INT32_MAX;
+        if (wd[u]) {
+          const int p = __builtin_ctzll(wd[u]);
+          x[u] = a.gf[(int64_t)cc[u] * BS + (BS - 1 - p)];
+          wd[u] &= wd[u] - 1;
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) acc = x[u] < acc ? x[u] : acc;
+    }
+  }
+  for (int off = BS; off < kWave; off <<= 1) {
+    const int32_t o = __shfl_xor(acc, off);
+    acc = o < acc ? o : acc;
+  }
+  if (grp == 0) a.mngf[R * BS + r] = acc;
+}
 
 template <int BS>
 __global__ __launch_bounds__(kBatchThreads) void k_batch(const BatchArgs a) {
   using W = word_t<BS>;
-  constexpr int G = kWave / BS;
-  const int tid = threadIdx.x;
+  int part = 0, nparts = 0;
+  if (!grid_join(a.sync, a.join_ticks, a.spin_ticks, part, nparts)) return;  // workgroup-uniform
+  part = __builtin_amdgcn_readfirstlane(part);
+  nparts = __builtin_amdgcn_readfirstlane(nparts);
+  constexpr int RG = row_group<BS>();
+  const uint64_t kLow = low_mask<BS>();
   const int lane = lane_id();
-  const int wave = tid / kWave;
-  const int grp = lane / BS, r = lane % BS;
-  const W* __restrict__ tiles = (const W*)a.tiles;
-  W* f = (W*)a.f;
-  W* nx = (W*)a.nx;
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kWave));
+  const int64_t gw = (int64_t)part * kBatchWaves + wave;
+  const int64_t nw = (int64_t)nparts * kBatchWaves;
+  const int64_t gt = (int64_t)part * kBatchThreads + threadIdx.x;
+  const int64_t nt = (int64_t)nparts * kBatchThreads;
+  const int64_t nbr = a.nblockrows, nu = nbr * BS;
+  const int64_t ngroups = (nbr + RG - 1) / RG;
+  W* f0 = (W*)a.f0;
+  W* f1 = (W*)a.f1;
   W* vis = (W*)a.vis;
-  __shared__ int s_g;
-  __shared__ unsigned long long s_cnt;
+  __shared__ int s_any;
+  // wave-private set-bit lists and accumulators of the bs 8/16 CC sweep
+  constexpr int kListWaves = BS <= 16 ? kBatchWaves : 1;
+  __shared__ uint32_t s_idx[kListWaves][kListCap];
+  __shared__ uint8_t s_row[kListWaves][kListCap];
+  __shared__ int32_t s_acc[kListWaves][RG * BS];
+  GridBarrier bar{a.sync, a.spin_ticks, (unsigned int)nparts, part == 0};
 
-  while (true) {
-    if (tid == 0) s_g = (int)atomicAdd(a.queue, 1u);
-    __syncthreads();
-    const int qi = s_g;
-    __syncthreads();
-    if (qi >= a.ngraphs) break;  // workgroup-uniform
-    const int g = a.order[qi];
-    const int32_t b0 = a.blk0[g], b1 = a.blk1[g];
-    const int64_t vb = a.vbase[g];
-    const int32_t n = a.n[g];
-    const int64_t oo = a.out[g];
-    const int32_t src = a.src[g];
-
-    // ------------------------------------------------------------ BFS
-    for (int64_t R = b0 + tid; R < b1; R += kBatchThreads) {
-      const int64_t s_union = vb + src;
-      const W w = (R == s_union / BS) ? (W)(1ull << (BS - 1 - (s_union % BS))) : (W)0;
-      f[R] = w;
-      vis[R] = w;
-    }
-    for (int64_t v = tid; v < n; v += kBatchThreads) a.level[oo + v] = (v == src) ? 0 : -1;
-    __syncthreads();
-    int32_t lvl = 0;
-    while (true) {
-      if (tid == 0) s_cnt = 0;
-      __syncthreads();
-      for (int64_t R = b0 + wave; R < b1; R += kBatchWaves) {  // one wave per block row
-        const int64_t rs = a.rowptr[R], re = a.rowptr[R + 1];
-        const W vw = vis[R];
-        int hit = 0;
-        for (int64_t t = rs + grp; t < re; t += G) {
-          const W fr = f[a.colind[t]];
-          if (fr != 0 && !hit && (W)(tiles[t * BS + r] & fr) != 0) hit = 1;
-        }
-        for (int off = BS; off < kWave; off <<= 1) hit |= __shfl_xor(hit, off);
-        const uint64_t bal = __ballot(hit);
-        const uint64_t word = brev_low<BS>(bal & low_mask<BS>()) & ~(uint64_t)vw & low_mask<BS>();
-        if (lane == 0) {
-          nx[R] = (W)word;
-          if (word) atomicAdd(&s_cnt, (unsigned long long)popcw(word));
-        }
+  // ------------------------------------------------------------ BFS
+  for (int64_t R = gt; R < nbr; R += nt) {
+    const int g = a.rowg[R];
+    const int64_t s = a.vbase[g] + a.src[g];
+    const W w = (R == s / BS) ? (W)(1ull << (BS - 1 - (s % BS))) : (W)0;
+    f0[R] = w;
+    vis[R] = w;
+  }
+  for (int64_t u = gt; u < nu; u += nt) {
+    const int g = a.rowg[u / BS];
+    const int64_t v = u - a.vbase[g];
+    if (v < a.n[g]) a.level[a.out[g] + v] = (v == a.src[g]) ? 0 : -1;
+  }
+  for (int64_t g = gt; g < a.ngraphs; g += nt) {
+    a.bfs_act[g] = 0;
+    a.cc_act[g] = 0;
+  }
+  if (!bar.sync()) return;
+  for (int32_t k = 0;; ++k) {
+    // frontier of level k in fk; the vertices found at level k+1 go to fn
+    const W* fk = (k & 1) ? f1 : f0;
+    W* fn = (k & 1) ? f0 : f1;
+    bar.open(&s_any);
+    int marked = -1;  // graph whose marker this wave already raised to k+1
+    for (int64_t q = gw; q < ngroups; q += nw) {
+      // lane j < nrow describes block row R0 + j
+      const int64_t R0 = q * RG;
+      const int nrow = nbr - R0 < RG ? (int)(nbr - R0) : RG;
+      int32_t l_rs = 0, l_re = 0, l_g = 0;
+      int64_t l_vw = 0, l_ob = 0;
+      bool sweep = false;
+      if (lane < nrow) {
+        const int64_t R = R0 + lane;
+        l_rs = a.rowptr[R];
+        l_re = a.rowptr[R + 1];
+        l_g = a.rowg[R];
+        l_vw = (int64_t)(uint64_t)vis[R];
+        const int64_t vb = a.vbase[l_g];
+        const int64_t valid = vb + a.n[l_g] - R * BS;
+        const uint64_t pad = valid >= BS ? 0ull : (kLow >> valid);
+        l_ob = a.out[l_g] - vb;
+        // skip: the graph's frontier is empty, or every vertex of the block
+        // row is visited (rows past the graph's last vertex count as visited)
+        sweep = a.bfs_act[l_g] >= k && (((uint64_t)l_vw | pad) & kLow) != kLow && l_re > l_rs;
+        if (!sweep) fn[R] = (W)0;
       }
-      __syncthreads();
-      const unsigned long long cnt = s_cnt;
-      if (cnt == 0) break;  // workgroup-uniform
-      ++lvl;
-      for (int64_t R = b0 + tid; R < b1; R += kBatchThreads) {
-        uint64_t w = (uint64_t)nx[R];
-        f[R] = (W)w;
-        if (w) {
-          vis[R] = (W)((uint64_t)vis[R] | w);
-          while (w) {
-            const int p = __builtin_ctzll(w);
-            w &= w - 1;
-            const int64_t v = R * BS + (BS - 1 - p) - vb;
-            if (v >= 0 && v < n) a.level[oo + v] = lvl;
+      uint64_t todo = __ballot(sweep);
+      while (todo) {
+        const int j = __builtin_ctzll(todo);
+        todo &= todo - 1;
+        const int64_t R = R0 + j;
+        const int32_t rs = __builtin_amdgcn_readlane(l_rs, j), re = __builtin_amdgcn_readlane(l_re, j);
+        const uint64_t vw = (uint64_t)readlane64(l_vw, j);
+        uint64_t rows;
+        if constexpr (BS <= 16) rows = bfs_row_lpt<BS>(a, fk, rs, re, lane);
+        else rows = bfs_row_rl<BS>(a, fk, rs, re, lane);
+        const uint64_t word = brev_low<BS>(rows) & ~vw & kLow;
+        if (lane == 0) fn[R] = (W)word;
+        if (word) {  // wave-uniform
+          const int g = __builtin_amdgcn_readlane(l_g, j);
+          if (lane == 0) {
+            vis[R] = (W)(vw | word);
+            if (g != marked) atomicMax(&a.bfs_act[g], k + 1);
+            s_any = 1;
           }
+          marked = g;
+          const int64_t ob = readlane64(l_ob, j);
+          if (lane < BS && ((word >> (BS - 1 - lane)) & 1)) a.level[ob + R * BS + lane] = k + 1;
         }
       }
-      __syncthreads();
     }
+    const int more = bar.close(&s_any);
+    if (more < 0) return;
+    if (!more) break;
+  }
 
-    // ------------------------------------------------------------- CC (FastSV)
-    for (int64_t v = tid; v < n; v += kBatchThreads) {
-      a.lab[vb + v] = (int32_t)(vb + v);
-      a.gf[vb + v] = (int32_t)(vb + v);
+  // ------------------------------------------------------------- CC (FastSV)
+  for (int64_t u = gt; u < nu; u += nt) {
+    a.lab[u] = (int32_t)u;
+    a.gf[u] = (int32_t)u;
+  }
+  if (!bar.sync()) return;
+  for (int32_t it = 0;; ++it) {
+    // a graph takes part in iteration it if it changed in iteration it-1
+    for (int64_t q = gw; q < ngroups; q += nw) {
+      const int64_t R0 = q * RG;
+      const int nrow = nbr - R0 < RG ? (int)(nbr - R0) : RG;
+      int32_t l_rs = 0, l_re = 0;
+      bool sweep = false;
+      if (lane < nrow) {
+        const int64_t R = R0 + lane;
+        l_rs = a.rowptr[R];
+        l_re = a.rowptr[R + 1];
+        sweep = a.cc_act[a.rowg[R]] >= it;
+      }
+      uint64_t todo = __ballot(sweep);
+      if constexpr (BS <= 16) {
+        if (todo) cc_group_lpt<BS, RG>(a, R0, nrow, l_rs, l_re, todo, lane, s_idx[wave], s_row[wave], s_acc[wave]);
+      } else {
+        while (todo) {
+          const int j = __builtin_ctzll(todo);
+          todo &= todo - 1;
+          const int32_t rs = __builtin_amdgcn_readlane(l_rs, j), re = __builtin_amdgcn_readlane(l_re, j);
+          cc_row_rl<BS>(a, R0 + j, rs, re, lane);
+        }
+      }
     }
-    __syncthreads();
-    while (true) {
-      // mngf[i] = min over neighbours of gf (one wave per block row)
-      for (int64_t R = b0 + wave; R < b1; R += kBatchWaves) {
-        const int64_t rs = a.rowptr[R], re = a.rowptr[R + 1];
-        int32_t acc = INT32_MAX;
-        for (int64_t t = rs + grp; t < re; t += G) {
-          const int32_t col = a.colind[t];
-          uint64_t wd = (uint64_t)tiles[t * BS + r];
-          while (wd) {
-            const int p = __builtin_ctzll(wd);
-            wd &= wd - 1;
-            const int32_t x = a.gf[(int64_t)col * BS + (BS - 1 - p)];
-            acc = x < acc ? x : acc;
-          }
-        }
-        for (int off = BS; off < kWave; off <<= 1) {
-          const int32_t o = __shfl_xor(acc, off);
-          acc = o < acc ? o : acc;
-        }
-        const int64_t i = R * BS + r;
-        if (grp == 0 && i - vb < n) a.mngf[i] = acc;
-      }
-      __syncthreads();
-      if (tid == 0) s_cnt = 0;
-      for (int64_t v = tid; v < n; v += kBatchThreads) {  // hooking + shortcut
-        const int64_t u = vb + v;
-        const int32_t m = a.mngf[u];
-        const int32_t fu = a.lab[u];
-        if (m < fu) atomicMin(&a.lab[fu], m);
-        const int32_t gu = a.gf[u];
-        atomicMin(&a.lab[u], m < gu ? m : gu);
-      }
-      __syncthreads();
-      for (int64_t v = tid; v < n; v += kBatchThreads) {  // grandparents
-        const int64_t u = vb + v;
+    if (!bar.sync()) return;
+    for (int64_t u = gt; u < nu; u += nt) {  // hooking + shortcut
+      const int g = a.rowg[u / BS];
+      if (a.cc_act[g] < it || u - a.vbase[g] >= a.n[g]) continue;
+      // labels only decrease, so a stale read of lab[u] is never below its
+      // value and a candidate not below it would leave it unchanged
+      const int32_t m = a.mngf[u];
+      const int32_t fu = a.lab[u];
+      if (m < fu) atomicMin(&a.lab[fu], m);
+      const int32_t gu = a.gf[u];
+      const int32_t cand = m < gu ? m : gu;
+      if (cand < fu) atomicMin(&a.lab[u], cand);
+    }
+    if (!bar.sync()) return;
+    bar.open(&s_any);
+    for (int64_t u = gt; u < nu; u += nt) {  // grandparents
+      const int g = a.rowg[u / BS];
+      bool changed = false;
+      if (a.cc_act[g] >= it && u - a.vbase[g] < a.n[g]) {
         const int32_t g2 = a.lab[a.lab[u]];
         if (g2 != a.gf[u]) {
           a.gf[u] = g2;
-          atomicAdd(&s_cnt, 1ull);
+          changed = true;
         }
       }
-      __syncthreads();
-      if (s_cnt == 0) break;  // workgroup-uniform; s_cnt is reset after the next barrier
-    }
-    while (true) {  // pointer jumping to stars
-      __syncthreads();
-      if (tid == 0) s_cnt = 0;
-      __syncthreads();
-      for (int64_t v = tid; v < n; v += kBatchThreads) {
-        const int64_t u = vb + v;
-        const int32_t p = a.lab[u], gp = a.lab[p];
-        if (gp != p) {
-          a.lab[u] = gp;
-          atomicAdd(&s_cnt, 1ull);
-        }
+      // mark each changed graph once per wave: its lowest changed lane does
+      uint64_t pend = __ballot(changed);
+      if (pend && lane == 0) s_any = 1;
+      while (pend) {
+        const int gl = __builtin_amdgcn_readlane(g, __builtin_ctzll(pend));
+        pend &= ~__ballot(changed && g == gl);
+        if (lane == 0) atomicMax(&a.cc_act[gl], it + 1);
       }
-      __syncthreads();
-      if (s_cnt == 0) break;
     }
-    for (int64_t v = tid; v < n; v += kBatchThreads) a.label[oo + v] = (int32_t)(a.lab[vb + v] - vb);
-    __syncthreads();
+    const int more = bar.close(&s_any);
+    if (more < 0) return;
+    if (!more) break;
+  }
+  while (true) {  // pointer jumping to stars
+    bar.open(&s_any);
+    for (int64_t u = gt; u < nu; u += nt) {
+      const int32_t p = a.lab[u], gp = a.lab[p];
+      if (gp != p) {
+        a.lab[u] = gp;
+        s_any = 1;
+      }
+    }
+    const int more = bar.close(&s_any);
+    if (more < 0) return;
+    if (!more) break;
+  }
+  for (int64_t u = gt; u < nu; u += nt) {
+    const int g = a.rowg[u / BS];
+    const int64_t v = u - a.vbase[g];
+    if (v < a.n[g]) a.label[a.out[g] + v] = (int32_t)(a.lab[u] - a.vbase[g]);
   }
 }
 
@@ -234,9 +701,9 @@ static void batch_release(gbo_batch* b) {
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
   gbo_bsr_free(&b->A);
-  for (void* p : {(void*)b->d_blk0, (void*)b->d_blk1, (void*)b->d_vbase, (void*)b->d_n, (void*)b->d_out,
-                  (void*)b->d_order, (void*)b->d_src, b->d_f, b->d_nx, b->d_vis, (void*)b->d_lab, (void*)b->d_gf,
-                  (void*)b->d_mngf, (void*)b->d_queue})
+  for (void* p : {(void*)b->d_rowg, (void*)b->d_vbase, (void*)b->d_n, (void*)b->d_out, (void*)b->d_src, b->d_f0,
+                  b->d_f1, b->d_vis, (void*)b->d_lab, (void*)b->d_gf, (void*)b->d_mngf, (void*)b->d_act,
+                  (void*)b->d_sync})
     if (p) (void)hipFree(p);
   delete b;
 }
@@ -248,40 +715,72 @@ static int upload(T** d, const std::vector<T>& h) {
   return GBO_OK;
 }
 
+// Launch-time constants: the grid is every workgroup that fits on the GPU at
+// once (the grid barrier needs all of them resident), at most one per
+// kBatchWaves block rows.
+template <int BS>
+static int batch_kernel_launch(const BatchArgs& a, hipStream_t st) {
+  int dev = 0, cus = 0, per_cu = 0;
+  GBO_HIP_CHECK(hipGetDevice(&dev));
+  GBO_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  GBO_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_batch<BS>, kBatchThreads, 0));
+  if (per_cu < 1) return set_error(GBO_EDEVICE, "gbo_batch_bfs_cc: kernel does not fit on a CU");
+  constexpr int RG = row_group<BS>();
+  const int64_t want = ((a.nblockrows + RG - 1) / RG + kBatchWaves - 1) / kBatchWaves;
+  const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(want, (int64_t)per_cu * cus));
+  k_batch<BS><<<grid, kBatchThreads, 0, st>>>(a);
+  return GBO_OK;
+}
+
 static int batch_launch(gbo_batch* b, int32_t* d_level, int32_t* d_label, hipStream_t st) {
   BatchArgs a{};
   a.rowptr = b->A.rowptr;
   a.colind = b->A.colind;
   a.tiles = b->A.tiles;
+  a.nblockrows = b->A.nblockrows;
   a.ngraphs = b->ngraphs;
-  a.blk0 = b->d_blk0;
-  a.blk1 = b->d_blk1;
+  a.rowg = b->d_rowg;
   a.vbase = b->d_vbase;
   a.n = b->d_n;
   a.out = b->d_out;
-  a.order = b->d_order;
   a.src = b->d_src;
-  a.f = b->d_f;
-  a.nx = b->d_nx;
+  a.f0 = b->d_f0;
+  a.f1 = b->d_f1;
   a.vis = b->d_vis;
   a.lab = b->d_lab;
   a.gf = b->d_gf;
   a.mngf = b->d_mngf;
+  a.bfs_act = b->d_act;
+  a.cc_act = b->d_act + b->ngraphs;
   a.level = d_level;
   a.label = d_label;
-  a.queue = b->d_queue;
-  int dev = 0, cus = 0;
+  a.sync = b->d_sync;
+  int dev = 0, khz = 0;
   GBO_HIP_CHECK(hipGetDevice(&dev));
-  GBO_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-  const unsigned grid = (unsigned)std::max(1, std::min(b->ngraphs, 2 * cus));
-  k_batch_reset<<<1, 1, 0, st>>>(b->d_queue);
+  GBO_HIP_CHECK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+  a.spin_ticks = (int64_t)std::max(khz, 1) * kBatchSpinMs;
+  a.join_ticks = (int64_t)std::max(khz, 1) * kBatchJoinUs / 1000;
+  k_batch_reset<<<1, 1, 0, st>>>(b->d_sync);
+  int rc = GBO_OK;
   switch (b->bs) {
-    case 8: k_batch<8><<<grid, kBatchThreads, 0, st>>>(a); break;
-    case 16: k_batch<16><<<grid, kBatchThreads, 0, st>>>(a); break;
-    case 32: k_batch<32><<<grid, kBatchThreads, 0, st>>>(a); break;
-    case 64: k_batch<64><<<grid, kBatchThreads, 0, st>>>(a); break;
+    case 8: rc = batch_kernel_launch<8>(a, st); break;
+    case 16: rc = batch_kernel_launch<16>(a, st); break;
+    case 32: rc = batch_kernel_launch<32>(a, st); break;
+    case 64: rc = batch_kernel_launch<64>(a, st); break;
   }
+  if (rc) return rc;
   GBO_LAUNCH_CHECK();
+  return GBO_OK;
+}
+
+// After the stream has drained: a grid barrier that timed out means the
+// results are incomplete.
+static int batch_check(gbo_batch* b) {
+  unsigned int abort = 0;
+  GBO_HIP_CHECK(hipMemcpy(&abort, &b->d_sync->abort, sizeof(abort), hipMemcpyDeviceToHost));
+  if (abort)
+    return set_error(GBO_EDEVICE, "gbo_batch_bfs_cc: grid barrier wait exceeded %d ms (workgroups not co-resident)",
+                     kBatchSpinMs);
   return GBO_OK;
 }
 
@@ -341,32 +840,27 @@ int gbo_batch_create(int bs, int32_t ngraphs, const int32_t* h_nrows, const int3
     batch_release(b);
     return rc;
   }
-  // largest graphs first (longest-processing-time order)
-  std::vector<int32_t> brp((size_t)b->A.nblockrows + 1);
-  GBO_HIP_CHECK(hipMemcpy(brp.data(), b->A.rowptr, sizeof(int32_t) * brp.size(), hipMemcpyDeviceToHost));
-  std::vector<int32_t> order(ngraphs);
-  std::iota(order.begin(), order.end(), 0);
-  std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
-    const int64_t tx = (int64_t)brp[blk1[x]] - brp[blk0[x]] + nn[x], ty = (int64_t)brp[blk1[y]] - brp[blk0[y]] + nn[y];
-    return tx > ty;
-  });
+  // graph of each block row
+  std::vector<int32_t> rowg((size_t)b->A.nblockrows);
+  for (int32_t g = 0; g < ngraphs; ++g) std::fill(rowg.begin() + blk0[g], rowg.begin() + blk1[g], g);
   b->nunion = nu;
   b->out_off = outo;
   std::vector<int64_t> oo(outo.begin(), outo.end() - 1);
-  if ((rc = upload(&b->d_blk0, blk0)) || (rc = upload(&b->d_blk1, blk1)) || (rc = upload(&b->d_vbase, vbase)) ||
-      (rc = upload(&b->d_n, nn)) || (rc = upload(&b->d_out, oo)) || (rc = upload(&b->d_order, order))) {
+  if ((rc = upload(&b->d_rowg, rowg)) || (rc = upload(&b->d_vbase, vbase)) || (rc = upload(&b->d_n, nn)) ||
+      (rc = upload(&b->d_out, oo))) {
     batch_release(b);
     return rc;
   }
   const size_t wb = (size_t)bs / 8 * ((size_t)b->A.nblockrows + 1);
   hipError_t e = hipSuccess;
-  e = e ? e : hipMalloc(&b->d_f, wb);
-  e = e ? e : hipMalloc(&b->d_nx, wb);
+  e = e ? e : hipMalloc(&b->d_f0, wb);
+  e = e ? e : hipMalloc(&b->d_f1, wb);
   e = e ? e : hipMalloc(&b->d_vis, wb);
   e = e ? e : hipMalloc((void**)&b->d_lab, sizeof(int32_t) * nu);
   e = e ? e : hipMalloc((void**)&b->d_gf, sizeof(int32_t) * nu);
   e = e ? e : hipMalloc((void**)&b->d_mngf, sizeof(int32_t) * nu);
-  e = e ? e : hipMalloc((void**)&b->d_queue, sizeof(unsigned int));
+  e = e ? e : hipMalloc((void**)&b->d_act, sizeof(int32_t) * 2 * ngraphs);
+  e = e ? e : hipMalloc((void**)&b->d_sync, sizeof(GridSync));
   e = e ? e : hipMalloc((void**)&b->d_src, sizeof(int32_t) * ngraphs);
   if (e != hipSuccess) {
     batch_release(b);
@@ -405,7 +899,7 @@ int gbo_batch_bfs_cc(gbo_batch* b, const int32_t* h_sources, int32_t* d_level, i
     int rc = batch_launch(b, d_level, d_label, st);
     if (rc) return rc;
     GBO_HIP_CHECK(hipStreamSynchronize(st));
-    return GBO_OK;
+    return batch_check(b);
   }
   if (b->exec && (b->graph_level != d_level || b->graph_label != d_label)) {
     (void)hipGraphExecDestroy(b->exec);
@@ -432,7 +926,7 @@ int gbo_batch_bfs_cc(gbo_batch* b, const int32_t* h_sources, int32_t* d_level, i
   }
   GBO_HIP_CHECK(hipGraphLaunch(b->exec, st));
   GBO_HIP_CHECK(hipStreamSynchronize(st));
-  return GBO_OK;
+  return batch_check(b);
 }
 
 void gbo_batch_free(gbo_batch* b) { batch_release(b); }

@@ -260,6 +260,17 @@ def test_graph_batch(gbo, po, bs):
     the same BFS levels and CC labels as per-graph plain runs and the oracle."""
     graphs = gbo.gen_er_batch(count=24, n_lo=50, n_hi=3000, d_lo=0.5, d_hi=6.0, seed=4)
     graphs.append(gbo.gen_rmat(10, 4, seed=6))
+    # ragged members: one isolated vertex, one edge, a 700-vertex path (many
+    # more BFS levels than the rest of the batch), n = 65 (a partial block row)
+    path = np.arange(699)
+    pr = np.concatenate([path, path + 1])
+    pc = np.concatenate([path + 1, path])
+    o = np.lexsort((pc, pr))
+    pr, pc = pr[o], pc[o]
+    graphs.insert(0, gbo.Csr(1, 1, np.array([0, 0]), np.zeros(0, np.int32)))
+    graphs.insert(3, gbo.Csr(2, 2, np.array([0, 1, 2]), np.array([1, 0])))
+    graphs.insert(7, gbo.Csr(700, 700, np.searchsorted(pr, np.arange(701)), pc))
+    graphs.append(gbo.gen_er_m(65, 100, seed=9))
     rng = np.random.default_rng(bs)
     src = np.array([rng.integers(0, g.nrows) for g in graphs], np.int32)
     B = gbo.GraphBatch(graphs, bs)
