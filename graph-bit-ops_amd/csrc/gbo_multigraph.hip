@@ -270,25 +270,39 @@ __device__ __forceinline__ void tile_words(const void* tiles, int64_t t, uint64_
   }
 }
 
-// Pull-BFS over the tiles [rs, re) of one block row, bs 8/16: lane per tile.
-// Column indices of 64 tiles are one coalesced load, their frontier words
-// one gather, and a tile is read only if its frontier word is nonzero.
-// Returns (wave-uniform) bit r set iff row r has a frontier neighbour.
-template <int BS>
-__device__ __forceinline__ uint64_t bfs_row_lpt(const BatchArgs& a, const word_t<BS>* fk, int32_t rs, int32_t re,
-                                                int lane) {
-  constexpr int NS = (BS == 8) ? 4 : 2;  // tiles per lane per round
-  constexpr int NWD = BS / 4 > 1 ? BS / 4 : 1;  // words per tile: 1 (bs 8) or 4 (bs 16)
+// Pull-BFS over the block rows R0 .. R0+nrow-1 whose bit is set in amask,
+// bs 8/16 (RG*bs = 128 vertices).  The rows' tiles are contiguous and are
+// taken lane per tile, 64*NS at a time, across row boundaries: column
+// indices are one coalesced load per slot, frontier words one gather, and a
+// tile is read only if its frontier word is nonzero.  Returns in gm[0..3]
+// (wave-uniform) the 128-bit mask of rows with a frontier neighbour, bit
+// j*bs + k = row k of block row R0+j.
+template <int BS, int RG>
+__device__ __forceinline__ void bfs_group_lpt(const BatchArgs& a, const word_t<BS>* fk, int nrow, int32_t l_rs,
+                                              int32_t l_re, uint64_t amask, int lane, uint32_t* gm) {
+  static_assert(RG * BS == 128, "group mask is 128 bits");
+  constexpr int NS = 4;
+  constexpr int NWD = BS / 4 > 1 ? BS / 4 : 1;
   const int32_t* __restrict__ colind = a.colind;
-  uint32_t rows = 0;
-  for (int32_t t0 = rs; t0 < re; t0 += NS * kWave) {
+#pragma unroll
+  for (int q = 0; q < 4; ++q) gm[q] = 0;
+  const int32_t gb = __builtin_amdgcn_readlane(l_rs, 0), ge = __builtin_amdgcn_readlane(l_re, nrow - 1);
+  int32_t rend[RG - 1];  // row ends (wave-uniform)
+#pragma unroll
+  for (int j = 0; j < RG - 1; ++j) rend[j] = j < nrow ? __builtin_amdgcn_readlane(l_re, j) : ge;
+  for (int32_t t0 = gb; t0 < ge; t0 += NS * kWave) {
     int32_t c[NS];
-    uint32_t f[NS];
+    int rr[NS];
 #pragma unroll
     for (int u = 0; u < NS; ++u) {
       const int32_t t = t0 + u * kWave + lane;
-      c[u] = t < re ? colind[t] : -1;
+      int r = 0;
+#pragma unroll
+      for (int j = 0; j < RG - 1; ++j) r += t >= rend[j];
+      rr[u] = r;
+      c[u] = (t < ge && ((amask >> r) & 1)) ? colind[t] : -1;
     }
+    uint32_t f[NS];
 #pragma unroll
     for (int u = 0; u < NS; ++u) f[u] = c[u] >= 0 ? (uint32_t)fk[c[u]] : 0u;
 #pragma unroll
@@ -296,17 +310,23 @@ __device__ __forceinline__ uint64_t bfs_row_lpt(const BatchArgs& a, const word_t
       if (f[u]) {
         uint64_t w[NWD];
         tile_words<BS>(a.tiles, t0 + u * kWave + lane, w);
+        uint32_t m;  // bit k: row k of the tile meets the frontier
         if constexpr (BS == 8) {
-          rows |= nz_bytes(w[0] & (0x0101010101010101ull * f[u]));
+          m = nz_bytes(w[0] & (0x0101010101010101ull * f[u]));
         } else {
           const uint64_t rep = 0x0001000100010001ull * f[u];
-#pragma unroll
-          for (int q = 0; q < 4; ++q) rows |= nz_shorts(w[q] & rep) << (4 * q);
+          m = nz_shorts(w[0] & rep) | (nz_shorts(w[1] & rep) << 4) | (nz_shorts(w[2] & rep) << 8) |
+              (nz_shorts(w[3] & rep) << 12);
         }
+        const int bit = rr[u] * BS;
+        const uint32_t v = m << (bit & 31);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) gm[q] |= (bit >> 5) == q ? v : 0u;
       }
     }
   }
-  return wave_or(rows);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) gm[q] = wave_or(gm[q]);
 }
 
 // Pull-BFS over one block row, bs 32/64: lane per tile for the column
@@ -574,15 +594,46 @@ __global__ __launch_bounds__(kBatchThreads) void k_batch(const BatchArgs a) {
         if (!sweep) fn[R] = (W)0;
       }
       uint64_t todo = __ballot(sweep);
+      if constexpr (BS <= 16) {
+        if (!todo) continue;
+        uint32_t gm[4];
+        bfs_group_lpt<BS, RG>(a, fk, nrow, l_rs, l_re, todo, lane, gm);
+        // lane j < nrow: block row R0 + j
+        uint64_t word = 0;
+        if (lane < nrow && ((todo >> lane) & 1)) {
+          const int bit = lane * BS;
+          uint32_t gq = gm[0];
+#pragma unroll
+          for (int q = 1; q < 4; ++q) gq = (bit >> 5) == q ? gm[q] : gq;
+          const uint64_t rows = (gq >> (bit & 31)) & kLow;
+          word = brev_low<BS>(rows) & ~(uint64_t)l_vw & kLow;
+          fn[R0 + lane] = (W)word;
+          if (word) vis[R0 + lane] = (W)((uint64_t)l_vw | word);
+        }
+        uint64_t pend = __ballot(word != 0);
+        if (!pend) continue;
+        if (lane == 0) s_any = 1;
+        for (uint64_t mk = pend; mk;) {  // each graph's marker once
+          const int gl = __builtin_amdgcn_readlane(l_g, __builtin_ctzll(mk));
+          mk &= ~__ballot(word != 0 && l_g == gl);
+          if (lane == 0 && gl != marked) atomicMax(&a.bfs_act[gl], k + 1);
+          marked = gl;
+        }
+        for (int p = lane; p < nrow * BS; p += kWave) {  // levels of the new vertices
+          const int j = p / BS, kk = p % BS;
+          const uint64_t wj = __shfl(word, j);
+          const int64_t obj = __shfl(l_ob, j);
+          if ((wj >> (BS - 1 - kk)) & 1) a.level[obj + (R0 + j) * BS + kk] = k + 1;
+        }
+        continue;
+      }
       while (todo) {
         const int j = __builtin_ctzll(todo);
         todo &= todo - 1;
         const int64_t R = R0 + j;
         const int32_t rs = __builtin_amdgcn_readlane(l_rs, j), re = __builtin_amdgcn_readlane(l_re, j);
         const uint64_t vw = (uint64_t)readlane64(l_vw, j);
-        uint64_t rows;
-        if constexpr (BS <= 16) rows = bfs_row_lpt<BS>(a, fk, rs, re, lane);
-        else rows = bfs_row_rl<BS>(a, fk, rs, re, lane);
+        const uint64_t rows = bfs_row_rl<BS>(a, fk, rs, re, lane);
         const uint64_t word = brev_low<BS>(rows) & ~vw & kLow;
         if (lane == 0) fn[R] = (W)word;
         if (word) {  // wave-uniform
