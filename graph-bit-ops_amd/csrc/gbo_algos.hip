@@ -133,10 +133,13 @@ static int bfs_t(const gbo_bsr* A, const gbo_bsr* Aout, int32_t source, int32_t*
   nx = f + std::max(nbr, 1);
   vis = nx + std::max(nbr, 1);
   GBO_HIP_CHECK(hipMallocAsync((void**)&d_cnt, sizeof(int64_t), st));
+  unsigned long long* slots = nullptr;
+  int rc = count_slots_create(&slots, st);
+  if (rc) return rc;
   int32_t* segoff = nullptr;
   if (Aout) GBO_HIP_CHECK(hipMallocAsync((void**)&segoff, sizeof(int32_t) * ((size_t)nbr + 1), st));
   PlanView pv;
-  int rc = acquire_plan(A, &pv, st);
+  rc = acquire_plan(A, &pv, st);
   if (rc) return rc;
   const unsigned blocks_n = (unsigned)((std::max<int64_t>(n, nbr) + 255) / 256);
   k_bfs_init<BS><<<blocks_n, 256, 0, st>>>(f, vis, d_level, nbr, n, source);
@@ -153,7 +156,7 @@ static int bfs_t(const gbo_bsr* A, const gbo_bsr* Aout, int32_t source, int32_t*
   a.chunk = pv.chunk;
   a.nchunks = pv.nchunks;
   a.variant = pv.variant;
-  a.count = d_cnt;
+  a.count_slots = slots;
   int32_t lvl = 0;
   int64_t fsize = 1;  // vertices in the current frontier
   if (sizes && max_levels > 0) sizes[0] = 1;
@@ -176,6 +179,8 @@ static int bfs_t(const gbo_bsr* A, const gbo_bsr* Aout, int32_t source, int32_t*
       a.out = nx;
       rc = spmv_launch(BS, OP_BFS, a, st);
       if (rc) break;
+      rc = count_fold(slots, d_cnt, st);
+      if (rc) break;
     }
     int64_t cnt = 0;
     GBO_HIP_CHECK(hipMemcpyAsync(&cnt, d_cnt, sizeof(int64_t), hipMemcpyDeviceToHost, st));
@@ -193,6 +198,7 @@ static int bfs_t(const gbo_bsr* A, const gbo_bsr* Aout, int32_t source, int32_t*
   W* base = std::min(std::min(f, nx), vis);
   (void)hipFreeAsync(base, st);
   (void)hipFreeAsync(d_cnt, st);
+  (void)hipFreeAsync(slots, st);
   if (segoff) (void)hipFreeAsync(segoff, st);
   GBO_HIP_CHECK(hipStreamSynchronize(st));
   return rc;
@@ -213,14 +219,16 @@ __global__ __launch_bounds__(256) void k_sssp_update(const uint32_t* __restrict_
       dist[i] = c;
       fr[i] = c;
       improved = 1;
-      const int64_t J = i / bs;
-      atomicOr(&active[J >> 6], 1ull << (J & 63));
     } else {
       fr[i] = kInfU32;
     }
   }
-  const uint64_t b = __ballot(improved);
-  if (lane_id() == 0 && b) atomicAdd(changed, (unsigned long long)__popcll(b));
+  // block columns of the improved vertices: the 64 vertices of a wave (64-
+  // aligned) cover 64/bs consecutive block columns inside one 64-bit word
+  const int64_t J = i / bs;
+  const uint64_t m = wave_or64(improved ? 1ull << (J & 63) : 0ull);
+  if (lane_id() == 0 && m) atomicOr(&active[((i - lane_id()) / bs) >> 6], m);
+  wave_flag(improved, changed);
 }
 
 __global__ void k_sssp_init(uint32_t* dist, uint32_t* fr, unsigned long long* active, int32_t n, int bs, int32_t s) {
@@ -290,7 +298,7 @@ __global__ __launch_bounds__(256) void k_cc_hook(const int32_t* __restrict__ mng
   atomicMin(&f[u], best);
 }
 
-// gf[u] = f[f[u]]; counts the vertices whose grandparent changed.
+// gf[u] = f[f[u]]; *changed = 1 if any vertex's grandparent changed.
 __global__ __launch_bounds__(256) void k_cc_grand(const int32_t* __restrict__ f, int32_t* __restrict__ gf, int32_t n,
                                                   unsigned long long* __restrict__ changed) {
   const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -302,8 +310,7 @@ __global__ __launch_bounds__(256) void k_cc_grand(const int32_t* __restrict__ f,
       ch = 1;
     }
   }
-  const uint64_t b = __ballot(ch);
-  if (lane_id() == 0 && b) atomicAdd(changed, (unsigned long long)__popcll(b));
+  wave_flag(ch, changed);
 }
 
 // f[u] = f[f[u]] until every vertex points at a root (final star shape).
@@ -317,8 +324,7 @@ __global__ __launch_bounds__(256) void k_cc_jump(int32_t* __restrict__ f, int32_
       ch = 1;
     }
   }
-  const uint64_t b = __ballot(ch);
-  if (lane_id() == 0 && b) atomicAdd(changed, (unsigned long long)__popcll(b));
+  wave_flag(ch, changed);
 }
 
 __global__ void k_iota(int32_t* a, int32_t* b, int32_t n) {

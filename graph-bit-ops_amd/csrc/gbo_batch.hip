@@ -120,14 +120,14 @@ __global__ __launch_bounds__(256) void k_ms_prefill(const MsArgs a) {
 }
 
 // V |= Fn; level[s][v] = lvl for every new (s, v); next activity mask;
-// number of new (s, v) pairs.
+// *count = 1 if any (s, v) is new.
 template <int KW>
 __global__ __launch_bounds__(256) void k_ms_update(const uint64_t* __restrict__ Fn, uint64_t* __restrict__ V,
                                                    int32_t* __restrict__ level, int32_t n, int32_t nsrc, int bs,
                                                    int32_t lvl, unsigned long long* __restrict__ active,
                                                    unsigned long long* __restrict__ count) {
   const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  uint64_t cnt = 0;
+  uint64_t cnt = 0, act = 0;
   if (v < n) {
     bool any = false;
 #pragma unroll
@@ -143,13 +143,13 @@ __global__ __launch_bounds__(256) void k_ms_update(const uint64_t* __restrict__ 
         level[(int64_t)(w * 64 + b) * n + v] = lvl;
       }
     }
-    if (any) {
-      const int64_t J = v / bs;
-      atomicOr(&active[J >> 6], 1ull << (J & 63));
-    }
+    if (any) act = 1ull << ((v / bs) & 63);
   }
-  for (int off = 1; off < kWave; off <<= 1) cnt += __shfl_xor(cnt, off);
-  if (lane_id() == 0 && cnt) atomicAdd(count, (unsigned long long)cnt);
+  // the 64 vertices of a wave (64-aligned) fall in one word of the block
+  // column mask
+  const uint64_t m = wave_or64(act);
+  if (lane_id() == 0 && m) atomicOr(&active[((v - lane_id()) / bs) >> 6], m);
+  wave_flag(cnt != 0, count);
 }
 
 template <int KW>

@@ -299,7 +299,7 @@ __global__ __launch_bounds__(256) void k_spmv(const SpmvArgs a) {
     R0 += kWave;
   }
   if constexpr (OP == OP_BFS) {
-    if (lane == 0 && added) atomicAdd((unsigned long long*)a.count, (unsigned long long)added);
+    if (lane == 0 && added) atomicAdd(&a.count_slots[c & (kCountSlots - 1)], (unsigned long long)added);
   }
 }
 
@@ -533,7 +533,7 @@ __global__ __launch_bounds__(256) void k_spmv_lpt(const SpmvArgs a) {
     R0 += kWave;
   }
   if constexpr (OP == OP_BFS) {
-    if (lane == 0 && added) atomicAdd((unsigned long long*)a.count, (unsigned long long)added);
+    if (lane == 0 && added) atomicAdd(&a.count_slots[c & (kCountSlots - 1)], (unsigned long long)added);
   }
 }
 
@@ -1033,7 +1033,10 @@ static int spmv_launch_t(const SpmvArgs& a, hipStream_t st) {
   // variant 3 (k_count8) exists for bs 8; bs 16 runs variant 2 for it
   const bool lean8 = (BS == 8 && OP == OP_COUNT && a.variant == 3 && aligned);
   const bool stream = (BS <= 16 && OP == OP_COUNT && (a.variant == 2 || a.variant == 3) && aligned && !lean8);
-  const bool lpt = (BS <= 16 && (OP == OP_COUNT || OP == OP_BFS) && a.variant == 1);
+  // variant 3 at bs 8 is the lean count kernel; its BFS step is the
+  // lane-per-tile kernel (faster than the row-group one at bs 8)
+  const bool lpt = (BS <= 16 && (OP == OP_COUNT || OP == OP_BFS) && a.variant == 1) ||
+                   (BS == 8 && OP == OP_BFS && a.variant == 3);
   const bool need_prefill = OP == OP_BFS || OP == OP_MINPLUS || OP == OP_MIN || (OP == OP_COUNT && !a.accumulate);
   if (need_prefill && a.nchunks > 1) {
     const int64_t n = (int64_t)(a.nchunks - 1) * BS;
@@ -1111,6 +1114,40 @@ static SpmvArgs base_args(const gbo_bsr* A, const PlanView& pv) {
 }
 
 // Runs one operation with a plan acquired (and released) around it.
+__global__ __launch_bounds__(256) void k_zero_u64(unsigned long long* p, int n) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0ull;
+}
+
+// One block: *out = sum of the slots; the slots are cleared for the next use.
+__global__ __launch_bounds__(256) void k_count_fold(unsigned long long* slots, int64_t* out) {
+  __shared__ unsigned long long sh[256];
+  unsigned long long s = 0;
+  for (int i = threadIdx.x; i < kCountSlots; i += 256) {
+    s += slots[i];
+    slots[i] = 0ull;
+  }
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = (int64_t)sh[0];
+}
+
+int count_slots_create(unsigned long long** slots, hipStream_t st) {
+  GBO_HIP_CHECK(hipMallocAsync((void**)slots, sizeof(unsigned long long) * kCountSlots, st));
+  k_zero_u64<<<kCountSlots / 256, 256, 0, st>>>(*slots, kCountSlots);
+  GBO_LAUNCH_CHECK();
+  return GBO_OK;
+}
+
+int count_fold(unsigned long long* slots, int64_t* d_count, hipStream_t st) {
+  k_count_fold<<<1, 256, 0, st>>>(slots, d_count);
+  GBO_LAUNCH_CHECK();
+  return GBO_OK;
+}
+
 template <class Fill>
 static int run_op(const gbo_bsr* A, int op, hipStream_t st, Fill fill) {
   PlanView pv;
@@ -1146,14 +1183,19 @@ int gbo_bsrbmv_bfs_step(const gbo_bsr* A, const void* d_frontier, const void* d_
   int rc = validate_bsr(A, 0, "gbo_bsrbmv_bfs_step");
   if (rc) return rc;
   if (!d_frontier || !d_visited || !d_next || !d_count) return set_error(GBO_EINVAL, "gbo_bsrbmv_bfs_step: null argument");
-  GBO_HIP_CHECK(hipMemsetAsync(d_count, 0, sizeof(int64_t), stream));
-  return run_op(A, OP_BFS, stream, [&](SpmvArgs& a) -> int {
+  unsigned long long* slots = nullptr;
+  rc = count_slots_create(&slots, stream);
+  if (rc) return rc;
+  rc = run_op(A, OP_BFS, stream, [&](SpmvArgs& a) -> int {
     a.xbits = d_frontier;
     a.visited = d_visited;
     a.out = d_next;
-    a.count = d_count;
+    a.count_slots = slots;
     return GBO_OK;
   });
+  if (rc == GBO_OK) rc = count_fold(slots, d_count, stream);
+  (void)hipFreeAsync(slots, stream);
+  return rc;
 }
 
 int gbo_bsrbmv_minplus(const gbo_bsr* A, const uint32_t* d_w, const int64_t* d_woff, const uint32_t* d_dist,

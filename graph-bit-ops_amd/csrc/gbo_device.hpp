@@ -40,6 +40,20 @@ __device__ __forceinline__ uint64_t brev_low(uint64_t x) {
   return __brevll(x) >> (64 - BS);
 }
 
+// OR of v over the wave (every lane gets it).
+__device__ __forceinline__ uint64_t wave_or64(uint64_t v) {
+  for (int off = kWave / 2; off >= 1; off >>= 1) v |= __shfl_xor(v, off);
+  return v;
+}
+
+// *flag = 1 if any lane of the wave passes `any`.  Termination flags are
+// only ever set to the same value, so this is a plain store after a plain
+// read instead of a read-modify-write: tens of thousands of waves do not
+// queue on one address.
+__device__ __forceinline__ void wave_flag(bool any, unsigned long long* flag) {
+  if (__ballot(any) && (threadIdx.x & (kWave - 1)) == 0 && *flag == 0ull) *flag = 1ull;
+}
+
 // Mask with the low BS bits set.
 template <int BS>
 __device__ __forceinline__ uint64_t low_mask() {

@@ -48,7 +48,7 @@ struct SpmvArgs {
   const uint32_t* w;    // MINPLUS weights in tile bit order (nullable)
   const int64_t* woff;  // MINPLUS per-tile weight offsets
   const void* xval;     // MINPLUS dist / SUM x / MIN x
-  int64_t* count;       // BFS: number of newly set bits
+  unsigned long long* count_slots;  // BFS: newly set bits, summed per slot (kCountSlots, zero on entry)
   double* partial;      // SUM: split-row partials [nchunks][2][bs]
   const uint64_t* active;  // MINPLUS: optional per-block-column activity bits (nullable)
   int accumulate;       // COUNT: add into y
@@ -56,5 +56,13 @@ struct SpmvArgs {
 };
 
 int spmv_launch(int bs, int op, const SpmvArgs& a, hipStream_t st);
+
+// BFS frontier-size counters.  Every wave adds its count of newly set bits
+// to slot (chunk index mod kCountSlots) instead of one shared counter, so
+// hundreds of thousands of waves do not serialise on a single address;
+// count_fold() sums the slots into one int64 and clears them again.
+constexpr int kCountSlots = 1024;
+int count_slots_create(unsigned long long** slots, hipStream_t st);
+int count_fold(unsigned long long* slots, int64_t* d_count, hipStream_t st);
 
 }  // namespace gbo

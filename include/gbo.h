@@ -182,8 +182,8 @@ int gbo_bsr_plan(gbo_bsr* A, int32_t chunk_tiles, gbo_stream_t stream);
  * step's loads in flight across block-row boundaries (COUNT only; BFS runs
  * variant 0; needs 16-byte-aligned tiles/colind, else variant 0 runs),
  * 3 = bs 8 COUNT with 32-bit chunk-relative indexing and byte-transposed
- * popcounts (4 tiles per v_bcnt) and a permlane/DPP row reduction (bs 16
- * runs variant 2).
+ * popcounts (4 tiles per v_bcnt) and a permlane/DPP row reduction; its BFS
+ * step runs variant 1 (bs 16 runs variant 2).
  * Results are identical; only speed differs (the autotuner picks).
  * gbo_bsr_create / gbo_bsr_create_dev set variant 3 for bs 8, 0 otherwise. */
 int gbo_bsr_plan_ex(gbo_bsr* A, int32_t chunk_tiles, int32_t variant, gbo_stream_t stream);
