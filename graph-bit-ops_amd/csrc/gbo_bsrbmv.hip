@@ -40,6 +40,16 @@ __device__ __forceinline__ void write_empty_row(const SpmvArgs& a, int32_t R) {
   }
 }
 
+// Tile row word for the value products (min-plus, sum, min).  At bs 32/64
+// the tiles are streamed once and far outweigh the gathered vector, so they
+// are loaded nontemporally to leave L2 to the gathers (measured 2-8 % faster
+// there; slightly slower at bs 8/16, which keep plain loads).
+template <int BS>
+__device__ __forceinline__ word_t<BS> ld_value_tile(const word_t<BS>* p) {
+  if constexpr (BS >= 32) return __builtin_nontemporal_load(p);
+  else return *p;
+}
+
 template <int BS, int OP>
 __global__ __launch_bounds__(256) void k_spmv(const SpmvArgs a) {
   using W = word_t<BS>;
@@ -164,7 +174,7 @@ __global__ __launch_bounds__(256) void k_spmv(const SpmvArgs a) {
             wo[u] = 0;
             if (tu < te) {
               cc[u] = colind[tu];
-              wd[u] = (uint64_t)tiles[tu * BS + r];
+              wd[u] = (uint64_t)ld_value_tile<BS>(&tiles[tu * BS + r]);
               if (a.w) wo[u] = a.woff[tu];
               // inactive source block column: the tile is skipped (group-uniform)
               if (a.active && !((a.active[cc[u] >> 6] >> (cc[u] & 63)) & 1ull)) wd[u] = 0;
@@ -230,7 +240,7 @@ __global__ __launch_bounds__(256) void k_spmv(const SpmvArgs a) {
           for (int u = 0; u < 4; ++u) {
             const int64_t tu = t + u * G;
             cc[u] = tu < te ? colind[tu] : 0;
-            wd[u] = tu < te ? (uint64_t)tiles[tu * BS + r] : 0ull;
+            wd[u] = tu < te ? (uint64_t)ld_value_tile<BS>(&tiles[tu * BS + r]) : 0ull;
           }
           while (wd[0] | wd[1] | wd[2] | wd[3]) {
             float v[4];
@@ -266,7 +276,7 @@ __global__ __launch_bounds__(256) void k_spmv(const SpmvArgs a) {
           for (int u = 0; u < 4; ++u) {
             const int64_t tu = t + u * G;
             cc[u] = tu < te ? colind[tu] : 0;
-            wd[u] = tu < te ? (uint64_t)tiles[tu * BS + r] : 0ull;
+            wd[u] = tu < te ? (uint64_t)ld_value_tile<BS>(&tiles[tu * BS + r]) : 0ull;
           }
           while (wd[0] | wd[1] | wd[2] | wd[3]) {
             int32_t v[4];  // the 4 gathers of a pass are issued before any is used
