@@ -202,16 +202,34 @@ def bench_multisource(gbo, A, g, nsrc=256, nplain=32, seed=3):
             "bit_exact_vs_plain": exact, "plain_sample": nplain, "bs": A.bs}
 
 
-def bench_graph_batch(gbo, count=512, bs=8, seed=4):
+def bench_graph_batch(gbo, count=512, bs=None, seed=4):
     """C4b: `count` ER graphs (n log-uniform in [1k, 64k], mean degree in
     [8, 32]) -- BFS + CC of all of them in one persistent hipGraph launch vs
-    per-graph plain gbo_bfs + gbo_cc."""
+    per-graph plain gbo_bfs + gbo_cc.  The block size is the autotuner's pick
+    for a BFS-step sweep of the block-diagonal union unless `bs` is given."""
     t = time.perf_counter()
     graphs = gbo.gen_er_batch(count=count, seed=seed)
     gen_s = time.perf_counter() - t
     nnz = sum(g.nnz for g in graphs)
     rng = np.random.default_rng(seed)
     src = np.array([rng.integers(0, g.nrows) for g in graphs], np.int32)
+    tuned = None
+    if bs is None:
+        nu = sum(g.nrows for g in graphs)
+        rp, ci, off, base = [np.zeros(1, np.int64)], [], 0, 0
+        for g in graphs:
+            rp.append(g.rowptr[1:].astype(np.int64) + off)
+            ci.append(g.colind + base)
+            off += g.nnz
+            base += g.nrows
+        d_rp = gbo.DeviceBuffer.from_numpy(np.concatenate(rp).astype(np.int32))
+        d_ci = gbo.DeviceBuffer.from_numpy(np.concatenate(ci).astype(np.int32))
+        tu = gbo.autotune(d_rp, d_ci, nu, nu, op=1)
+        d_rp.free()
+        d_ci.free()
+        bs = tu.bs
+        tuned = {"op": "bfs step over the union", "bs": tu.bs, "chunk": tu.chunk, "variant": tu.variant,
+                 "ms": round(tu.ms, 4)}
     t = time.perf_counter()
     B = gbo.GraphBatch(graphs, bs)
     pack_s = time.perf_counter() - t
@@ -235,6 +253,7 @@ def bench_graph_batch(gbo, count=512, bs=8, seed=4):
         A.free()
     B.free()
     return {"graphs": count, "vertices": int(sum(g.nrows for g in graphs)), "nnz": int(nnz), "bs": bs,
+            "autotuned": tuned,
             "gen_s": round(gen_s, 1), "pack_s": round(pack_s, 2), "batched_ms": round(tb * 1e3, 2),
             "plain_ms": round(tp * 1e3, 1), "speedup": round(tp / tb, 2), "bit_exact_vs_plain": exact}
 

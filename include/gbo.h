@@ -321,11 +321,17 @@ int gbo_autotune(const int32_t* d_rowptr, const int32_t* d_colind, int32_t nrows
  * block-diagonal bit-BSR (each graph's first vertex aligned to bs).  One
  * persistent launch -- captured in a hipGraph when use_graph != 0 and
  * replayed by later calls with the same output buffers -- runs BFS from
- * h_sources[g] and FastSV connected components on every graph, each graph
- * on one workgroup with no host synchronisation between iterations.
- * Outputs are concatenated per graph (offsets from gbo_batch_offsets):
- * d_level as gbo_bfs, d_label = smallest graph-local vertex id of the
- * component, identical to per-graph gbo_bfs / gbo_cc runs.  No reference
+ * h_sources[g] and FastSV connected components on every graph with no host
+ * synchronisation between iterations: all graphs' block rows are swept
+ * together by the whole GPU, level by level, separated by grid barriers
+ * among the workgroups that are resident; each graph stops where a
+ * one-graph run would.  The rows given are read as in gbo_bfs reads AT
+ * (pull): for undirected graphs pass the symmetric CSR; connected
+ * components assume symmetric graphs.  Outputs are concatenated per graph
+ * (offsets from gbo_batch_offsets): d_level as gbo_bfs, d_label = smallest
+ * graph-local vertex id of the component, identical to per-graph gbo_bfs /
+ * gbo_cc runs.  A grid barrier wait beyond 5 s (workgroups that never got
+ * to run) ends the launch and returns GBO_EDEVICE.  No reference
  * counterpart (the reference runs one graph per process). */
 typedef struct gbo_batch gbo_batch;
 int gbo_batch_create(int bs, int32_t ngraphs, const int32_t* h_nrows, const int32_t* const* h_rowptr,
