@@ -286,19 +286,31 @@ __global__ __launch_bounds__(256) void k_ppr_fixup(const PprArgs A) {
 // `cols` <= 64 columns; mode 0: sum |Y - P|, mode 1: sum L * (Y - P),
 // mode 2: sum |Y|.  Each block writes 64 partials; k_colred_final adds
 // them in block order.
+constexpr int kColredBlocks = 2048;  // fixed, so the reduction order is the same on every GPU
+
 __global__ __launch_bounds__(256) void k_colred(const float* __restrict__ Y, const float* __restrict__ P,
                                                 const float* __restrict__ L, int32_t n, int32_t ld, int32_t cols,
                                                 int mode, double* __restrict__ part) {
   const int col = threadIdx.x & 63, rsub = threadIdx.x >> 6;
-  double acc = 0.0;
+  // four rows per step into four accumulators (loads of a step in flight
+  // together), combined in a fixed order: deterministic for a fixed grid
+  double acc4[4] = {0.0, 0.0, 0.0, 0.0};
   if (col < cols) {
-    for (int64_t i = (int64_t)blockIdx.x * 4 + rsub; i < n; i += (int64_t)gridDim.x * 4) {
-      const int64_t o = i * ld + col;
-      if (mode == 0) acc += fabs((double)Y[o] - (double)P[o]);
-      else if (mode == 1) acc += (double)L[o] * ((double)Y[o] - (double)P[o]);
-      else acc += fabs((double)Y[o]);
+    const int64_t S = (int64_t)gridDim.x * 4;
+    for (int64_t i0 = (int64_t)blockIdx.x * 4 + rsub; i0 < n; i0 += 4 * S) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int64_t i = i0 + q * S;
+        if (i < n) {
+          const int64_t o = i * ld + col;
+          if (mode == 0) acc4[q] += fabs((double)Y[o] - (double)P[o]);
+          else if (mode == 1) acc4[q] += (double)L[o] * ((double)Y[o] - (double)P[o]);
+          else acc4[q] += fabs((double)Y[o]);
+        }
+      }
     }
   }
+  const double acc = (acc4[0] + acc4[1]) + (acc4[2] + acc4[3]);
   __shared__ double sh[256];
   sh[threadIdx.x] = acc;
   __syncthreads();
@@ -439,7 +451,7 @@ static int spmm(const gbo_bsr* M, PprPlan& P, int32_t K, const float* X, const f
 // Column-wise reduction over [n][K] in column blocks of 64 (deterministic).
 static int colreduce(const float* Y, const float* P, const float* L, int32_t n, int32_t K, int mode, double* d_part,
                      double* d_out, double* h_out, hipStream_t st) {
-  constexpr int nb = 256;
+  constexpr int nb = kColredBlocks;
   for (int32_t k0 = 0; k0 < K; k0 += 64) {
     const int32_t cols = std::min(64, K - k0);
     k_colred<<<nb, 256, 0, st>>>(Y + k0, P ? P + k0 : nullptr, L ? L + k0 : nullptr, n, K, cols, mode, d_part);
@@ -481,7 +493,7 @@ int gbo_ppr_forward(const gbo_bsr* AT, const int32_t* d_outdeg, int32_t K, float
   double* dpart = nullptr;
   GBO_HIP_CHECK(hipMallocAsync((void**)&inv, sizeof(float) * std::max(n, 1), st));
   GBO_HIP_CHECK(hipMallocAsync((void**)&q, sizeof(float) * std::max<int64_t>(nk, 1), st));
-  GBO_HIP_CHECK(hipMallocAsync((void**)&dpart, sizeof(double) * (256 * 64 + 64), st));
+  GBO_HIP_CHECK(hipMallocAsync((void**)&dpart, sizeof(double) * (kColredBlocks * 64 + 64), st));
   k_invdeg<<<(n + 255) / 256, 256, 0, st>>>(d_outdeg, inv, n);
   GBO_LAUNCH_CHECK();
   PprPlan P;
@@ -490,7 +502,7 @@ int gbo_ppr_forward(const gbo_bsr* AT, const int32_t* d_outdeg, int32_t K, float
   if (rc == GBO_OK) GBO_HIP_CHECK(hipMemcpyAsync(d_p, d_s, sizeof(float) * nk, hipMemcpyDeviceToDevice, st));
   std::vector<double> delta(K), snorm(K);
   // the change is measured relative to ||s_k||_1 (1 for a distribution)
-  if (rc == GBO_OK) rc = colreduce(d_s, nullptr, nullptr, n, K, 2, dpart, dpart + 256 * 64, snorm.data(), st);
+  if (rc == GBO_OK) rc = colreduce(d_s, nullptr, nullptr, n, K, 2, dpart, dpart + kColredBlocks * 64, snorm.data(), st);
   int32_t it = 0;
   double dmax = 0.0;
   float* cur = d_p;
@@ -498,7 +510,7 @@ int gbo_ppr_forward(const gbo_bsr* AT, const int32_t* d_outdeg, int32_t K, float
   while (rc == GBO_OK && it < max_iter) {
     rc = spmm(AT, P, K, cur, inv, nullptr, alpha, 1.0f - alpha, d_s, nxt, st);
     if (rc) break;
-    rc = colreduce(nxt, cur, nullptr, n, K, 0, dpart, dpart + 256 * 64, delta.data(), st);
+    rc = colreduce(nxt, cur, nullptr, n, K, 0, dpart, dpart + kColredBlocks * 64, delta.data(), st);
     if (rc) break;
     ++it;
     std::swap(cur, nxt);
@@ -538,7 +550,7 @@ int gbo_ppr_adjoint(const gbo_bsr* A, const gbo_bsr* AT, const int32_t* d_outdeg
   GBO_HIP_CHECK(hipMallocAsync((void**)&inv, sizeof(float) * std::max(n, 1), st));
   GBO_HIP_CHECK(hipMallocAsync((void**)&lam, sizeof(float) * std::max<int64_t>(nk, 1) * 2, st));
   q = lam + nk;
-  GBO_HIP_CHECK(hipMallocAsync((void**)&dpart, sizeof(double) * (256 * 64 + 64), st));
+  GBO_HIP_CHECK(hipMallocAsync((void**)&dpart, sizeof(double) * (kColredBlocks * 64 + 64), st));
   k_invdeg<<<(n + 255) / 256, 256, 0, st>>>(d_outdeg, inv, n);
   GBO_LAUNCH_CHECK();
   PprPlan PA, PT;
@@ -547,7 +559,7 @@ int gbo_ppr_adjoint(const gbo_bsr* A, const gbo_bsr* AT, const int32_t* d_outdeg
   if (rc == GBO_OK) GBO_HIP_CHECK(hipMemcpyAsync(lam, d_w, sizeof(float) * nk, hipMemcpyDeviceToDevice, st));
   std::vector<double> delta(K), ga(K), wnorm(K);
   // the change is measured relative to ||w_k||_1, the scale of lambda_k
-  if (rc == GBO_OK) rc = colreduce(d_w, nullptr, nullptr, n, K, 2, dpart, dpart + 256 * 64, wnorm.data(), st);
+  if (rc == GBO_OK) rc = colreduce(d_w, nullptr, nullptr, n, K, 2, dpart, dpart + kColredBlocks * 64, wnorm.data(), st);
   int32_t it = 0;
   float* cur = lam;
   float* nxt = q;
@@ -556,7 +568,7 @@ int gbo_ppr_adjoint(const gbo_bsr* A, const gbo_bsr* AT, const int32_t* d_outdeg
     // lambda' = alpha * D^-1 A lambda + w
     rc = spmm(A, PA, K, cur, nullptr, inv, alpha, 1.0f, d_w, nxt, st);
     if (rc) break;
-    rc = colreduce(nxt, cur, nullptr, n, K, 0, dpart, dpart + 256 * 64, delta.data(), st);
+    rc = colreduce(nxt, cur, nullptr, n, K, 0, dpart, dpart + kColredBlocks * 64, delta.data(), st);
     if (rc) break;
     ++it;
     std::swap(cur, nxt);
@@ -569,7 +581,7 @@ int gbo_ppr_adjoint(const gbo_bsr* A, const gbo_bsr* AT, const int32_t* d_outdeg
     GBO_LAUNCH_CHECK();
     // dL/dalpha = sum_k lambda_k . (P^T p_k - s_k)
     rc = spmm(AT, PT, K, d_p, inv, nullptr, 1.0f, 0.0f, nullptr, nxt, st);
-    if (rc == GBO_OK) rc = colreduce(nxt, d_s, cur, n, K, 1, dpart, dpart + 256 * 64, ga.data(), st);
+    if (rc == GBO_OK) rc = colreduce(nxt, d_s, cur, n, K, 1, dpart, dpart + kColredBlocks * 64, ga.data(), st);
     if (rc == GBO_OK && h_grad_alpha) {
       double s = 0.0;
       for (double v : ga) s += v;
