@@ -996,6 +996,180 @@ __global__ __launch_bounds__(256) void k_count8(const SpmvArgs a) {
 
 // Initializes the outputs of split rows (rows that begin before the chunk
 // that lists them) to the identity of the combining atomic.
+// bs 8 min / min-plus, lane per tile (plan variant 3).  A wave takes its
+// chunk 64 block rows (a window) at a time and the window's tiles 64 per
+// round (one per lane: more tiles per lane made the kernel register-bound
+// and slower), lane per tile across row boundaries; a 6-step shuffle search over
+// the window's row ends gives each tile's row.  A wave prefix sum of the
+// tiles' popcounts lists every set bit (target vertex, source vertex, weight
+// index) in a wave-private LDS list, so the value gathers are spread evenly
+// over the lanes (four per lane, issued together) however the bits fall in
+// the tiles, instead of one dependent pass per set bit of each lane's row
+// byte.  Each gathered candidate is folded into its vertex with an LDS
+// atomic min (signed values as x ^ 0x80000000, an order-preserving map to
+// unsigned); rows wholly inside the chunk are stored, split rows combined
+// with global atomics after the prefill, as in k_spmv.
+constexpr int kV8Cap = 4 * kWave;  // list entries per wave and pass
+
+template <int OP>
+__global__ __launch_bounds__(256) void k_value8(const SpmvArgs a) {
+  static_assert(OP == OP_MIN || OP == OP_MINPLUS, "min / min-plus only");
+  constexpr bool PLUS = OP == OP_MINPLUS;
+  constexpr int NS = 1;  // tiles per lane per round
+  constexpr uint32_t kFlip = 0x80000000u;
+  __shared__ uint32_t s_idx[4][kV8Cap];
+  __shared__ uint32_t s_widx[4][PLUS ? kV8Cap : 1];
+  __shared__ uint16_t s_dst[4][kV8Cap];
+  __shared__ uint32_t s_acc[4][kWave * 8];
+  const int lane = lane_id();
+  const int wv = threadIdx.x / kWave;
+  const int c = wave_index();
+  if (c >= a.nchunks) return;  // wave-uniform; no workgroup barriers below
+  const int64_t cb = (int64_t)c * a.chunk;
+  const int64_t ce = min(cb + (int64_t)a.chunk, a.T);
+  const bool last = (c == a.nchunks - 1);
+  const uint64_t* __restrict__ tl = (const uint64_t*)a.tiles;
+  const int32_t* __restrict__ colind = a.colind;
+  const uint32_t* __restrict__ xv = (const uint32_t*)a.xval;
+  const bool weighted = PLUS && a.w != nullptr;
+  uint32_t* idx = s_idx[wv];
+  uint32_t* widx = s_widx[wv];
+  uint16_t* dst = s_dst[wv];
+  uint32_t* acc = s_acc[wv];
+
+  for (int32_t R0 = a.plan[c]; R0 < a.nbr;) {
+    const int32_t Rl = R0 + lane;
+    int32_t rsl = 0, rel = 0;
+    if (Rl < a.nbr) {
+      rsl = a.rowptr[Rl];
+      rel = a.rowptr[Rl + 1];
+    }
+    const bool belong = (Rl < a.nbr) && (last || (int64_t)rsl < ce);
+    const int nb = __popcll(__ballot(belong));  // rows R0 .. R0+nb-1 (a prefix)
+    if (nb == 0) break;
+    if (belong && rel == rsl) write_empty_row<8, OP>(a, Rl);
+    const int32_t relx = belong ? rel : INT32_MAX;
+    const int64_t w0 = max((int64_t)__builtin_amdgcn_readlane(rsl, 0), cb);
+    const int64_t w1 = min((int64_t)__builtin_amdgcn_readlane(rel, nb - 1), ce);
+    for (int k = lane; k < kWave * 8; k += kWave) acc[k] = 0xFFFFFFFFu;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    for (int64_t t0 = w0; t0 < w1; t0 += NS * kWave) {
+      uint64_t w[NS];
+      uint32_t cv[NS], wb[NS];
+      int jr[NS];
+      int cnt = 0;
+#pragma unroll
+      for (int u = 0; u < NS; ++u) {
+        const int64_t t = t0 + u * kWave + lane;
+        w[u] = 0;
+        cv[u] = 0;
+        wb[u] = 0;
+        if (t < w1) {
+          w[u] = tl[t];
+          const int32_t col = colind[t];
+          cv[u] = (uint32_t)col * 8u;
+          if (weighted) wb[u] = (uint32_t)a.woff[t];
+          // inactive source block column: the tile is skipped
+          if (PLUS && a.active && !((a.active[col >> 6] >> (col & 63)) & 1ull)) w[u] = 0;
+        }
+        // window row of tile t (all lanes take part in the shuffles)
+        const int64_t tc = t < w1 ? t : w1 - 1;
+        int j = 0;
+#pragma unroll
+        for (int step = kWave / 2; step >= 1; step >>= 1)
+          if ((int64_t)__shfl(relx, j + step - 1) <= tc) j += step;
+        jr[u] = j * 8;
+        cnt += __popcll(w[u]);
+      }
+      int incl = cnt;  // inclusive prefix sum of the popcounts over the lanes
+#pragma unroll
+      for (int off = 1; off < kWave; off <<= 1) {
+        const int o = __shfl_up(incl, off);
+        if (lane >= off) incl += o;
+      }
+      const int excl = incl - cnt;
+      const int tot = __builtin_amdgcn_readlane(incl, kWave - 1);
+      for (int base = 0; base < tot; base += kV8Cap) {
+        int e = excl;
+#pragma unroll
+        for (int u = 0; u < NS; ++u) {
+          uint64_t ww = w[u];
+          while (ww) {
+            const int b = __builtin_ctzll(ww);  // row b/8, column 7 - b%8
+            ww &= ww - 1;
+            if (e >= base && e < base + kV8Cap) {
+              idx[e - base] = cv[u] + (uint32_t)(7 - (b & 7));
+              dst[e - base] = (uint16_t)(jr[u] + (b >> 3));
+              if (weighted) {
+                // weights are in tile order: rows above first, then this
+                // row's columns left to right
+                const int r8 = (b >> 3) * 8;
+                const uint32_t below = (uint32_t)__popcll(w[u] & ((1ull << r8) - 1ull));
+                const uint32_t row = (uint32_t)((w[u] >> r8) & 0xFFu);
+                widx[e - base] = wb[u] + below + (uint32_t)__popc(row >> ((b & 7) + 1));
+              }
+            }
+            ++e;
+          }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        const int nl = tot - base < kV8Cap ? tot - base : kV8Cap;
+        uint32_t v[kV8Cap / kWave];
+        int d[kV8Cap / kWave];
+#pragma unroll
+        for (int q = 0; q < kV8Cap / kWave; ++q) {
+          const int e2 = q * kWave + lane;
+          v[q] = 0xFFFFFFFFu;
+          d[q] = 0;
+          if (e2 < nl) {
+            d[q] = dst[e2];
+            const uint32_t x = xv[idx[e2]];
+            if constexpr (PLUS) {
+              const uint32_t wt = weighted ? a.w[widx[e2]] : 1u;
+              v[q] = (x > kInfU32 - wt) ? kInfU32 : x + wt;
+            } else {
+              v[q] = x ^ kFlip;
+            }
+          }
+        }
+#pragma unroll
+        for (int q = 0; q < kV8Cap / kWave; ++q)
+          if (q * kWave + lane < nl) atomicMin(&acc[d[q]], v[q]);
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    // results of the window's rows that have tiles: lane per (row, tile row)
+    for (int p0 = 0; p0 < nb * 8; p0 += kWave) {
+      const int p = p0 + lane;
+      const int j = p >> 3;
+      const int32_t rs = __shfl(rsl, j & (kWave - 1)), re = __shfl(rel, j & (kWave - 1));
+      const int64_t i = (int64_t)(R0 + j) * 8 + (p & 7);
+      if (p < nb * 8 && re > rs && i < a.nrows) {
+        const bool full = (rs >= cb) && (re <= ce);
+        const uint32_t m = acc[p];
+        if constexpr (PLUS) {
+          uint32_t* out = (uint32_t*)a.out;
+          if (full) {
+            const uint32_t dd = xv[i];
+            out[i] = m < dd ? m : dd;
+          } else if (m != kInfU32) {
+            atomicMin(&out[i], m);
+          }
+        } else {
+          int32_t* y = (int32_t*)a.out;
+          const int32_t val = (int32_t)(m ^ kFlip);
+          if (full) y[i] = val;
+          else if (val != INT32_MAX) atomicMin(&y[i], val);
+        }
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    if (nb < kWave) break;
+    R0 += kWave;
+  }
+}
+
 template <int BS, int OP>
 __global__ __launch_bounds__(256) void k_prefill_split(const SpmvArgs a) {
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1070,6 +1244,10 @@ static int spmv_launch_t(const SpmvArgs& a, hipStream_t st) {
     }
   } else if constexpr (BS <= 16 && OP == OP_BFS) {
     if (lpt) k_spmv_lpt<BS, OP><<<(unsigned)blocks, 256, 0, st>>>(a);
+    else k_spmv<BS, OP><<<(unsigned)blocks, 256, 0, st>>>(a);
+  } else if constexpr (BS == 8 && (OP == OP_MIN || OP == OP_MINPLUS)) {
+    // variant 3 (the bs 8 default): the set-bit list kernel
+    if (a.variant == 3) k_value8<OP><<<(unsigned)blocks, 256, 0, st>>>(a);
     else k_spmv<BS, OP><<<(unsigned)blocks, 256, 0, st>>>(a);
   } else {
     k_spmv<BS, OP><<<(unsigned)blocks, 256, 0, st>>>(a);

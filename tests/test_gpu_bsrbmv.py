@@ -101,11 +101,15 @@ def test_bfs_step(gbo, po, graph_cases, bs, chunk, variant):
 
 
 @pytest.mark.parametrize("bs", BS)
-@pytest.mark.parametrize("chunk", CHUNKS)
-def test_minplus(gbo, po, graph_cases, bs, chunk):
+@pytest.mark.parametrize("chunk", COUNT_CHUNKS)
+@pytest.mark.parametrize("variant", (0, 3))
+def test_minplus(gbo, po, graph_cases, bs, chunk, variant):
+    """Variant 3 at bs 8 is the set-bit list kernel (k_value8)."""
+    if variant and bs != 8:
+        pytest.skip("variant 3 changes the min-plus kernel at bs 8 only")
     rng = np.random.default_rng(100 + bs)
     for name, g in graph_cases:
-        A, (rp, ci, tw) = packed(gbo, g, bs, chunk)
+        A, (rp, ci, tw) = packed(gbo, g, bs, chunk, variant)
         dist = rng.integers(0, 1000, g.nrows).astype(np.uint32)
         dist[rng.random(g.nrows) < 0.3] = gbo.INF_U32
         dist[rng.random(g.nrows) < 0.01] = gbo.INF_U32 - 3  # saturation
@@ -132,11 +136,14 @@ def test_sum_f32(gbo, po, graph_cases, bs, chunk):
 
 
 @pytest.mark.parametrize("bs", BS)
-@pytest.mark.parametrize("chunk", CHUNKS)
-def test_min_i32(gbo, po, graph_cases, bs, chunk):
+@pytest.mark.parametrize("chunk", COUNT_CHUNKS)
+@pytest.mark.parametrize("variant", (0, 3))
+def test_min_i32(gbo, po, graph_cases, bs, chunk, variant):
+    if variant and bs != 8:
+        pytest.skip("variant 3 changes the min kernel at bs 8 only")
     rng = np.random.default_rng(300 + bs)
     for name, g in graph_cases:
-        A, (rp, ci, tw) = packed(gbo, g, bs, chunk)
+        A, (rp, ci, tw) = packed(gbo, g, bs, chunk, variant)
         x = rng.integers(-2**31, 2**31 - 1, g.ncols).astype(np.int32)
         assert np.array_equal(gbo.bsrbmv_min_i32(A, x), po.min_i32(bs, g.nrows, rp, ci, tw, x)), name
 
