@@ -157,6 +157,12 @@ static int bfs_t(const gbo_bsr* A, const gbo_bsr* Aout, int32_t source, int32_t*
   a.nchunks = pv.nchunks;
   a.variant = pv.variant;
   a.count_slots = slots;
+  // bs 8 variant 3: levels while fewer than half the vertices are visited
+  // take the dense step (count kernel + pack); later ones skip visited rows
+  int32_t* dense_counts = nullptr;
+  if (BS == 8 && pv.variant == 3)
+    GBO_HIP_CHECK(hipMallocAsync((void**)&dense_counts, sizeof(int32_t) * 8 * ((size_t)nbr + 1), st));
+  int64_t visited = 1;
   int32_t lvl = 0;
   int64_t fsize = 1;  // vertices in the current frontier
   if (sizes && max_levels > 0) sizes[0] = 1;
@@ -177,6 +183,7 @@ static int bfs_t(const gbo_bsr* A, const gbo_bsr* Aout, int32_t source, int32_t*
       a.xbits = f;
       a.visited = vis;
       a.out = nx;
+      a.bfs_counts = (dense_counts && visited < (int64_t)n / 2) ? dense_counts : nullptr;
       rc = spmv_launch(BS, OP_BFS, a, st);
       if (rc) break;
       rc = count_fold(slots, d_cnt, st);
@@ -188,6 +195,7 @@ static int bfs_t(const gbo_bsr* A, const gbo_bsr* Aout, int32_t source, int32_t*
     if (cnt == 0) break;
     ++lvl;
     fsize = cnt;
+    visited += cnt;
     if (sizes && lvl < max_levels) sizes[lvl] = cnt;
     k_bfs_update<BS><<<(unsigned)((nbr + 255) / 256), 256, 0, st>>>(nx, vis, d_level, nbr, n, lvl);
     GBO_LAUNCH_CHECK();
@@ -199,6 +207,7 @@ static int bfs_t(const gbo_bsr* A, const gbo_bsr* Aout, int32_t source, int32_t*
   (void)hipFreeAsync(base, st);
   (void)hipFreeAsync(d_cnt, st);
   (void)hipFreeAsync(slots, st);
+  if (dense_counts) (void)hipFreeAsync(dense_counts, st);
   if (segoff) (void)hipFreeAsync(segoff, st);
   GBO_HIP_CHECK(hipStreamSynchronize(st));
   return rc;

@@ -1208,6 +1208,29 @@ __global__ __launch_bounds__(256) void k_sum_fixup(const SpmvArgs a) {
   if (i < a.nrows) ((float*)a.out)[i] = (float)((double)a.scale * s + (double)a.bias);
 }
 
+// Dense bs 8 BFS step, second half: next[R] = rows of block row R whose
+// frontier-neighbour count is nonzero, minus the visited ones; new bits are
+// counted into the slots.
+__global__ __launch_bounds__(256) void k_bfs_pack8(const int32_t* __restrict__ cnt, const uint8_t* __restrict__ vis,
+                                                   uint8_t* __restrict__ next, int32_t nbr, int32_t nrows,
+                                                   unsigned long long* __restrict__ slots) {
+  const int64_t R = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t word = 0;
+  if (R < nbr) {
+    const int4* p = (const int4*)(cnt + R * 8);
+    const int4 lo = p[0], hi = p[1];
+    const int32_t c8[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+#pragma unroll
+    for (int r = 0; r < 8; ++r)
+      if (R * 8 + r < nrows && c8[r] > 0) word |= 1u << (7 - r);
+    word &= ~(uint32_t)vis[R];
+    next[R] = (uint8_t)word;
+  }
+  uint64_t added = (uint64_t)__popc(word);
+  for (int off = 1; off < kWave; off <<= 1) added += __shfl_xor(added, off);
+  if (lane_id() == 0 && added) atomicAdd(&slots[(R / kWave) & (kCountSlots - 1)], (unsigned long long)added);
+}
+
 template <int BS, int OP>
 static int spmv_launch_t(const SpmvArgs& a, hipStream_t st) {
   // variants 2 and 3 (streamed) read 16- or 8-byte vectors of tiles and
@@ -1221,6 +1244,21 @@ static int spmv_launch_t(const SpmvArgs& a, hipStream_t st) {
   // lane-per-tile kernel (faster than the row-group one at bs 8)
   const bool lpt = (BS <= 16 && (OP == OP_COUNT || OP == OP_BFS) && a.variant == 1) ||
                    (BS == 8 && OP == OP_BFS && a.variant == 3);
+  if constexpr (BS == 8 && OP == OP_BFS) {
+    if (a.variant == 3 && a.bfs_counts && aligned) {
+      // dense step: the lean count kernel gives each row's number of
+      // frontier neighbours (every tile is read; no visited-row skip)
+      SpmvArgs c = a;
+      c.out = a.bfs_counts;
+      c.accumulate = 0;
+      int rc = spmv_launch_t<8, OP_COUNT>(c, st);
+      if (rc) return rc;
+      k_bfs_pack8<<<(unsigned)(((int64_t)a.nbr + 255) / 256), 256, 0, st>>>(
+          a.bfs_counts, (const uint8_t*)a.visited, (uint8_t*)a.out, a.nbr, a.nrows, a.count_slots);
+      GBO_LAUNCH_CHECK();
+      return GBO_OK;
+    }
+  }
   const bool need_prefill = OP == OP_BFS || OP == OP_MINPLUS || OP == OP_MIN || (OP == OP_COUNT && !a.accumulate);
   if (need_prefill && a.nchunks > 1) {
     const int64_t n = (int64_t)(a.nchunks - 1) * BS;
@@ -1374,14 +1412,20 @@ int gbo_bsrbmv_bfs_step(const gbo_bsr* A, const void* d_frontier, const void* d_
   unsigned long long* slots = nullptr;
   rc = count_slots_create(&slots, stream);
   if (rc) return rc;
+  int32_t* counts = nullptr;
   rc = run_op(A, OP_BFS, stream, [&](SpmvArgs& a) -> int {
     a.xbits = d_frontier;
     a.visited = d_visited;
     a.out = d_next;
     a.count_slots = slots;
+    if (A->bs == 8 && a.variant == 3) {
+      GBO_HIP_CHECK(hipMallocAsync((void**)&counts, sizeof(int32_t) * 8 * ((size_t)A->nblockrows + 1), stream));
+      a.bfs_counts = counts;
+    }
     return GBO_OK;
   });
   if (rc == GBO_OK) rc = count_fold(slots, d_count, stream);
+  if (counts) (void)hipFreeAsync(counts, stream);
   (void)hipFreeAsync(slots, stream);
   return rc;
 }

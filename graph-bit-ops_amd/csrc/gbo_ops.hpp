@@ -49,6 +49,8 @@ struct SpmvArgs {
   const int64_t* woff;  // MINPLUS per-tile weight offsets
   const void* xval;     // MINPLUS dist / SUM x / MIN x
   unsigned long long* count_slots;  // BFS: newly set bits, summed per slot (kCountSlots, zero on entry)
+  int32_t* bfs_counts;  // BFS, bs 8 variant 3 (nullable): int32 scratch of nbr*8; when set the step runs
+                        // the count kernel into it and packs the rows with a nonzero count (dense step)
   double* partial;      // SUM: split-row partials [nchunks][2][bs]
   const uint64_t* active;  // MINPLUS: optional per-block-column activity bits (nullable)
   int accumulate;       // COUNT: add into y

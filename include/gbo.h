@@ -183,7 +183,11 @@ int gbo_bsr_plan(gbo_bsr* A, int32_t chunk_tiles, gbo_stream_t stream);
  * variant 0; needs 16-byte-aligned tiles/colind, else variant 0 runs),
  * 3 = bs 8 COUNT with 32-bit chunk-relative indexing and byte-transposed
  * popcounts (4 tiles per v_bcnt) and a permlane/DPP row reduction; its BFS
- * step runs variant 1 (bs 16 runs variant 2).
+ * step is dense (that count kernel gives each row's number of frontier
+ * neighbours, a pack kernel keeps the nonzero unvisited rows; every tile is
+ * read, so gbo_bfs takes it while fewer than half the vertices are visited
+ * and variant 1 after), and min / min-plus list the tiles' set bits in LDS
+ * so the value gathers spread over all lanes (bs 16 runs variant 2).
  * Results are identical; only speed differs (the autotuner picks).
  * gbo_bsr_create / gbo_bsr_create_dev set variant 3 for bs 8, 0 otherwise. */
 int gbo_bsr_plan_ex(gbo_bsr* A, int32_t chunk_tiles, int32_t variant, gbo_stream_t stream);

@@ -83,9 +83,11 @@ def test_count_dense_bruteforce(gbo):
 
 
 @pytest.mark.parametrize("bs", BS)
-@pytest.mark.parametrize("chunk", CHUNKS)
-@pytest.mark.parametrize("variant", (0, 1))
+@pytest.mark.parametrize("chunk", COUNT_CHUNKS)
+@pytest.mark.parametrize("variant", (0, 1, 3))
 def test_bfs_step(gbo, po, graph_cases, bs, chunk, variant):
+    """Variant 3 at bs 8 is the dense step (count kernel, then the rows with
+    a nonzero count packed and the visited ones removed)."""
     if variant and bs > 16:
         pytest.skip("kernel variants exist for bs 8/16 only")
     rng = np.random.default_rng(bs + chunk)
