@@ -166,3 +166,23 @@ def test_autotune(gbo, po):
     xb = gbo.pack_vec(t.bs, x)
     rp, ci, tw = A.host_arrays()
     assert np.array_equal(gbo.bsrbmv_count(A, xb), po.count(t.bs, g.nrows, rp, ci, tw, xb))
+
+
+def test_autotune_bfs_step(gbo, po):
+    """BFS-step tuning (bs 8 includes the dense variant 3): the picked plan's
+    step equals the oracle's."""
+    g = gbo.gen_rmat(12, 16, seed=2)
+    d_rp = gbo.DeviceBuffer.from_numpy(g.rowptr)
+    d_ci = gbo.DeviceBuffer.from_numpy(g.colind)
+    t = gbo.autotune(d_rp, d_ci, g.nrows, g.ncols, 1)
+    assert t.bs in (8, 16, 32, 64) and t.chunk > 0 and t.ms > 0
+    assert t.variant in ((0, 1, 3) if t.bs == 8 else (0, 1) if t.bs == 16 else (0,))
+    A = gbo.BsrMatrix.from_csr(g, t.bs, plan=False)
+    A.plan(t.chunk, t.variant)
+    rng = np.random.default_rng(5)
+    f = gbo.pack_vec(t.bs, (rng.random(g.ncols) < 0.3).astype(np.float32))
+    v = gbo.pack_vec(t.bs, (rng.random(g.nrows) < 0.3).astype(np.float32))
+    rp, ci, tw = A.host_arrays()
+    nx, c = gbo.bsrbmv_bfs_step(A, f, v)
+    onx, oc = po.bfs_step(t.bs, g.nrows, rp, ci, tw, f, v)
+    assert np.array_equal(nx, onx) and c == oc
