@@ -994,33 +994,32 @@ __global__ __launch_bounds__(256) void k_count8(const SpmvArgs a) {
   if (open) finish_row();  // a row that continues past the chunk: atomic partial
 }
 
-// Initializes the outputs of split rows (rows that begin before the chunk
-// that lists them) to the identity of the combining atomic.
-// bs 8 min / min-plus, lane per tile (plan variant 3).  A wave takes its
+// bs 8/16 min / min-plus, lane per tile (plan variant 3).  A wave takes its
 // chunk 64 block rows (a window) at a time and the window's tiles 64 per
 // round (one per lane: more tiles per lane made the kernel register-bound
-// and slower), lane per tile across row boundaries; a 6-step shuffle search over
-// the window's row ends gives each tile's row.  A wave prefix sum of the
-// tiles' popcounts lists every set bit (target vertex, source vertex, weight
-// index) in a wave-private LDS list, so the value gathers are spread evenly
-// over the lanes (four per lane, issued together) however the bits fall in
-// the tiles, instead of one dependent pass per set bit of each lane's row
-// byte.  Each gathered candidate is folded into its vertex with an LDS
-// atomic min (signed values as x ^ 0x80000000, an order-preserving map to
-// unsigned); rows wholly inside the chunk are stored, split rows combined
-// with global atomics after the prefill, as in k_spmv.
-constexpr int kV8Cap = 4 * kWave;  // list entries per wave and pass
+// and slower), lane per tile across row boundaries; a 6-step shuffle search
+// over the window's row ends gives each tile's row.  A wave prefix sum of
+// the tiles' popcounts lists every set bit (target vertex, source vertex,
+// weight index) in a wave-private LDS list, so the value gathers are spread
+// evenly over the lanes (four per lane, issued together) however the bits
+// fall in the tiles, instead of one dependent pass per set bit of each
+// lane's row word.  Each gathered candidate is folded into its vertex with
+// an LDS atomic min (signed values as x ^ 0x80000000, an order-preserving
+// map to unsigned); rows wholly inside the chunk are stored, split rows
+// combined with global atomics after the prefill, as in k_spmv.
+constexpr int kValCap = 4 * kWave;  // list entries per wave and pass
 
-template <int OP>
-__global__ __launch_bounds__(256) void k_value8(const SpmvArgs a) {
+template <int BS, int OP>
+__global__ __launch_bounds__(256) void k_value_lpt(const SpmvArgs a) {
+  static_assert(BS == 8 || BS == 16, "lane-per-tile tiles of 8 or 32 bytes");
   static_assert(OP == OP_MIN || OP == OP_MINPLUS, "min / min-plus only");
   constexpr bool PLUS = OP == OP_MINPLUS;
-  constexpr int NS = 1;  // tiles per lane per round
+  constexpr int NWD = BS == 8 ? 1 : 4;  // 64-bit words per tile
   constexpr uint32_t kFlip = 0x80000000u;
-  __shared__ uint32_t s_idx[4][kV8Cap];
-  __shared__ uint32_t s_widx[4][PLUS ? kV8Cap : 1];
-  __shared__ uint16_t s_dst[4][kV8Cap];
-  __shared__ uint32_t s_acc[4][kWave * 8];
+  __shared__ uint32_t s_idx[4][kValCap];
+  __shared__ uint32_t s_widx[4][PLUS ? kValCap : 1];
+  __shared__ uint16_t s_dst[4][kValCap];
+  __shared__ uint32_t s_acc[4][kWave * BS];
   const int lane = lane_id();
   const int wv = threadIdx.x / kWave;
   const int c = wave_index();
@@ -1028,7 +1027,6 @@ __global__ __launch_bounds__(256) void k_value8(const SpmvArgs a) {
   const int64_t cb = (int64_t)c * a.chunk;
   const int64_t ce = min(cb + (int64_t)a.chunk, a.T);
   const bool last = (c == a.nchunks - 1);
-  const uint64_t* __restrict__ tl = (const uint64_t*)a.tiles;
   const int32_t* __restrict__ colind = a.colind;
   const uint32_t* __restrict__ xv = (const uint32_t*)a.xval;
   const bool weighted = PLUS && a.w != nullptr;
@@ -1047,40 +1045,48 @@ __global__ __launch_bounds__(256) void k_value8(const SpmvArgs a) {
     const bool belong = (Rl < a.nbr) && (last || (int64_t)rsl < ce);
     const int nb = __popcll(__ballot(belong));  // rows R0 .. R0+nb-1 (a prefix)
     if (nb == 0) break;
-    if (belong && rel == rsl) write_empty_row<8, OP>(a, Rl);
+    if (belong && rel == rsl) write_empty_row<BS, OP>(a, Rl);
     const int32_t relx = belong ? rel : INT32_MAX;
     const int64_t w0 = max((int64_t)__builtin_amdgcn_readlane(rsl, 0), cb);
     const int64_t w1 = min((int64_t)__builtin_amdgcn_readlane(rel, nb - 1), ce);
-    for (int k = lane; k < kWave * 8; k += kWave) acc[k] = 0xFFFFFFFFu;
+    for (int k = lane; k < kWave * BS; k += kWave) acc[k] = 0xFFFFFFFFu;
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    for (int64_t t0 = w0; t0 < w1; t0 += NS * kWave) {
-      uint64_t w[NS];
-      uint32_t cv[NS], wb[NS];
-      int jr[NS];
+    for (int64_t t0 = w0; t0 < w1; t0 += kWave) {
+      const int64_t t = t0 + lane;
+      uint64_t w[NWD];
+#pragma unroll
+      for (int q = 0; q < NWD; ++q) w[q] = 0;
+      uint32_t cv = 0, wb = 0;
+      if (t < w1) {
+        if constexpr (BS == 8) {
+          w[0] = ((const uint64_t*)a.tiles)[t];
+        } else {  // word q holds rows 4q..4q+3 as 16-bit fields
+          const uint4* tp = (const uint4*)a.tiles + 2 * t;
+          const uint4 lo = tp[0], hi = tp[1];
+          w[0] = lo.x | ((uint64_t)lo.y << 32);
+          w[1] = lo.z | ((uint64_t)lo.w << 32);
+          w[2] = hi.x | ((uint64_t)hi.y << 32);
+          w[3] = hi.z | ((uint64_t)hi.w << 32);
+        }
+        const int32_t col = colind[t];
+        cv = (uint32_t)col * BS;
+        if (weighted) wb = (uint32_t)a.woff[t];
+        // inactive source block column: the tile is skipped
+        if (PLUS && a.active && !((a.active[col >> 6] >> (col & 63)) & 1ull)) {
+#pragma unroll
+          for (int q = 0; q < NWD; ++q) w[q] = 0;
+        }
+      }
+      // window row of tile t (all lanes take part in the shuffles)
+      const int64_t tc = t < w1 ? t : w1 - 1;
+      int j = 0;
+#pragma unroll
+      for (int step = kWave / 2; step >= 1; step >>= 1)
+        if ((int64_t)__shfl(relx, j + step - 1) <= tc) j += step;
+      const int jr = j * BS;
       int cnt = 0;
 #pragma unroll
-      for (int u = 0; u < NS; ++u) {
-        const int64_t t = t0 + u * kWave + lane;
-        w[u] = 0;
-        cv[u] = 0;
-        wb[u] = 0;
-        if (t < w1) {
-          w[u] = tl[t];
-          const int32_t col = colind[t];
-          cv[u] = (uint32_t)col * 8u;
-          if (weighted) wb[u] = (uint32_t)a.woff[t];
-          // inactive source block column: the tile is skipped
-          if (PLUS && a.active && !((a.active[col >> 6] >> (col & 63)) & 1ull)) w[u] = 0;
-        }
-        // window row of tile t (all lanes take part in the shuffles)
-        const int64_t tc = t < w1 ? t : w1 - 1;
-        int j = 0;
-#pragma unroll
-        for (int step = kWave / 2; step >= 1; step >>= 1)
-          if ((int64_t)__shfl(relx, j + step - 1) <= tc) j += step;
-        jr[u] = j * 8;
-        cnt += __popcll(w[u]);
-      }
+      for (int q = 0; q < NWD; ++q) cnt += __popcll(w[q]);
       int incl = cnt;  // inclusive prefix sum of the popcounts over the lanes
 #pragma unroll
       for (int off = 1; off < kWave; off <<= 1) {
@@ -1089,35 +1095,41 @@ __global__ __launch_bounds__(256) void k_value8(const SpmvArgs a) {
       }
       const int excl = incl - cnt;
       const int tot = __builtin_amdgcn_readlane(incl, kWave - 1);
-      for (int base = 0; base < tot; base += kV8Cap) {
+      for (int base = 0; base < tot; base += kValCap) {
         int e = excl;
+        uint32_t before = 0;  // set bits in the tile's earlier words
 #pragma unroll
-        for (int u = 0; u < NS; ++u) {
-          uint64_t ww = w[u];
+        for (int q = 0; q < NWD; ++q) {
+          uint64_t ww = w[q];
           while (ww) {
-            const int b = __builtin_ctzll(ww);  // row b/8, column 7 - b%8
+            const int b = __builtin_ctzll(ww);
             ww &= ww - 1;
-            if (e >= base && e < base + kV8Cap) {
-              idx[e - base] = cv[u] + (uint32_t)(7 - (b & 7));
-              dst[e - base] = (uint16_t)(jr[u] + (b >> 3));
+            if (e >= base && e < base + kValCap) {
+              // row and bit of the row word (column bs-1-bit) of bit b of word q
+              const int fs = BS == 8 ? 3 : 4;        // log2 of the row field width
+              const int row = (BS == 8 ? 0 : 4 * q) + (b >> fs);
+              const int bit = b & (BS - 1);
+              idx[e - base] = cv + (uint32_t)(BS - 1 - bit);
+              dst[e - base] = (uint16_t)(jr + row);
               if (weighted) {
                 // weights are in tile order: rows above first, then this
                 // row's columns left to right
-                const int r8 = (b >> 3) * 8;
-                const uint32_t below = (uint32_t)__popcll(w[u] & ((1ull << r8) - 1ull));
-                const uint32_t row = (uint32_t)((w[u] >> r8) & 0xFFu);
-                widx[e - base] = wb[u] + below + (uint32_t)__popc(row >> ((b & 7) + 1));
+                const int sh = (b >> fs) << fs;
+                const uint32_t below = before + (uint32_t)__popcll(w[q] & ((1ull << sh) - 1ull));
+                const uint32_t rw = (uint32_t)((w[q] >> sh) & ((1ull << BS) - 1ull));
+                widx[e - base] = wb + below + (uint32_t)__popc(rw >> (bit + 1));
               }
             }
             ++e;
           }
+          before += (uint32_t)__popcll(w[q]);
         }
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        const int nl = tot - base < kV8Cap ? tot - base : kV8Cap;
-        uint32_t v[kV8Cap / kWave];
-        int d[kV8Cap / kWave];
+        const int nl = tot - base < kValCap ? tot - base : kValCap;
+        uint32_t v[kValCap / kWave];
+        int d[kValCap / kWave];
 #pragma unroll
-        for (int q = 0; q < kV8Cap / kWave; ++q) {
+        for (int q = 0; q < kValCap / kWave; ++q) {
           const int e2 = q * kWave + lane;
           v[q] = 0xFFFFFFFFu;
           d[q] = 0;
@@ -1133,19 +1145,19 @@ __global__ __launch_bounds__(256) void k_value8(const SpmvArgs a) {
           }
         }
 #pragma unroll
-        for (int q = 0; q < kV8Cap / kWave; ++q)
+        for (int q = 0; q < kValCap / kWave; ++q)
           if (q * kWave + lane < nl) atomicMin(&acc[d[q]], v[q]);
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
       }
     }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     // results of the window's rows that have tiles: lane per (row, tile row)
-    for (int p0 = 0; p0 < nb * 8; p0 += kWave) {
+    for (int p0 = 0; p0 < nb * BS; p0 += kWave) {
       const int p = p0 + lane;
-      const int j = p >> 3;
+      const int j = p / BS;
       const int32_t rs = __shfl(rsl, j & (kWave - 1)), re = __shfl(rel, j & (kWave - 1));
-      const int64_t i = (int64_t)(R0 + j) * 8 + (p & 7);
-      if (p < nb * 8 && re > rs && i < a.nrows) {
+      const int64_t i = (int64_t)(R0 + j) * BS + (p % BS);
+      if (p < nb * BS && re > rs && i < a.nrows) {
         const bool full = (rs >= cb) && (re <= ce);
         const uint32_t m = acc[p];
         if constexpr (PLUS) {
@@ -1170,6 +1182,8 @@ __global__ __launch_bounds__(256) void k_value8(const SpmvArgs a) {
   }
 }
 
+// Initializes the outputs of split rows (rows that begin before the chunk
+// that lists them) to the identity of the combining atomic.
 template <int BS, int OP>
 __global__ __launch_bounds__(256) void k_prefill_split(const SpmvArgs a) {
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1237,9 +1251,10 @@ static int spmv_launch_t(const SpmvArgs& a, hipStream_t st) {
   // column indices: they need 16-byte-aligned arrays, otherwise variant 0
   // runs.
   const bool aligned = ((((uintptr_t)a.tiles) | ((uintptr_t)a.colind)) & 15) == 0;
-  // variant 3 (k_count8) exists for bs 8; bs 16 runs variant 2 for it
+  // variant 3: bs 8 count is k_count8; bs 16 count runs variant 0 (its
+  // fastest count kernel)
   const bool lean8 = (BS == 8 && OP == OP_COUNT && a.variant == 3 && aligned);
-  const bool stream = (BS <= 16 && OP == OP_COUNT && (a.variant == 2 || a.variant == 3) && aligned && !lean8);
+  const bool stream = (BS <= 16 && OP == OP_COUNT && a.variant == 2 && aligned);
   // variant 3 at bs 8 is the lean count kernel; its BFS step is the
   // lane-per-tile kernel (faster than the row-group one at bs 8)
   const bool lpt = (BS <= 16 && (OP == OP_COUNT || OP == OP_BFS) && a.variant == 1) ||
@@ -1283,9 +1298,9 @@ static int spmv_launch_t(const SpmvArgs& a, hipStream_t st) {
   } else if constexpr (BS <= 16 && OP == OP_BFS) {
     if (lpt) k_spmv_lpt<BS, OP><<<(unsigned)blocks, 256, 0, st>>>(a);
     else k_spmv<BS, OP><<<(unsigned)blocks, 256, 0, st>>>(a);
-  } else if constexpr (BS == 8 && (OP == OP_MIN || OP == OP_MINPLUS)) {
-    // variant 3 (the bs 8 default): the set-bit list kernel
-    if (a.variant == 3) k_value8<OP><<<(unsigned)blocks, 256, 0, st>>>(a);
+  } else if constexpr (BS <= 16 && (OP == OP_MIN || OP == OP_MINPLUS)) {
+    // variant 3 (the bs 8/16 default): the set-bit list kernel
+    if (a.variant == 3 && aligned) k_value_lpt<BS, OP><<<(unsigned)blocks, 256, 0, st>>>(a);
     else k_spmv<BS, OP><<<(unsigned)blocks, 256, 0, st>>>(a);
   } else {
     k_spmv<BS, OP><<<(unsigned)blocks, 256, 0, st>>>(a);

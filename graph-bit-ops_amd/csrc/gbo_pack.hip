@@ -433,7 +433,9 @@ int gbo_bsr_create_dev(int bs, int dir, const int32_t* d_rp, const int32_t* d_ci
     gbo_bsr_free(&A);
     return rc;
   }
-  A.plan_variant = (bs == 8) ? 3 : 0;  // the fastest bs 8 count kernel on R-MAT s22
+  // variant 3: the fastest bs 8 count kernel on R-MAT s22 and, at bs 8/16,
+  // the set-bit list min / min-plus (bs 16 count runs its variant 0)
+  A.plan_variant = (bs <= 16) ? 3 : 0;
   *out = A;
   return GBO_OK;
 }

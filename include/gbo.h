@@ -186,10 +186,11 @@ int gbo_bsr_plan(gbo_bsr* A, int32_t chunk_tiles, gbo_stream_t stream);
  * step is dense (that count kernel gives each row's number of frontier
  * neighbours, a pack kernel keeps the nonzero unvisited rows; every tile is
  * read, so gbo_bfs takes it while fewer than half the vertices are visited
- * and variant 1 after), and min / min-plus list the tiles' set bits in LDS
- * so the value gathers spread over all lanes (bs 16 runs variant 2).
+ * and variant 1 after); at bs 8 and 16, min / min-plus list the tiles' set
+ * bits in LDS so the value gathers spread over all lanes (bs 16 COUNT and
+ * BFS run variant 0).
  * Results are identical; only speed differs (the autotuner picks).
- * gbo_bsr_create / gbo_bsr_create_dev set variant 3 for bs 8, 0 otherwise. */
+ * gbo_bsr_create / gbo_bsr_create_dev set variant 3 for bs 8/16, 0 otherwise. */
 int gbo_bsr_plan_ex(gbo_bsr* A, int32_t chunk_tiles, int32_t variant, gbo_stream_t stream);
 void gbo_bsr_plan_free(gbo_bsr* A);
 /* Per-nonzero values in tile bit order (row-major, MSB-first inside a row

@@ -106,9 +106,9 @@ def test_bfs_step(gbo, po, graph_cases, bs, chunk, variant):
 @pytest.mark.parametrize("chunk", COUNT_CHUNKS)
 @pytest.mark.parametrize("variant", (0, 3))
 def test_minplus(gbo, po, graph_cases, bs, chunk, variant):
-    """Variant 3 at bs 8 is the set-bit list kernel (k_value8)."""
-    if variant and bs != 8:
-        pytest.skip("variant 3 changes the min-plus kernel at bs 8 only")
+    """Variant 3 at bs 8/16 is the set-bit list kernel (k_value_lpt)."""
+    if variant and bs > 16:
+        pytest.skip("variant 3 changes the min-plus kernel at bs 8/16 only")
     rng = np.random.default_rng(100 + bs)
     for name, g in graph_cases:
         A, (rp, ci, tw) = packed(gbo, g, bs, chunk, variant)
@@ -141,8 +141,8 @@ def test_sum_f32(gbo, po, graph_cases, bs, chunk):
 @pytest.mark.parametrize("chunk", COUNT_CHUNKS)
 @pytest.mark.parametrize("variant", (0, 3))
 def test_min_i32(gbo, po, graph_cases, bs, chunk, variant):
-    if variant and bs != 8:
-        pytest.skip("variant 3 changes the min kernel at bs 8 only")
+    if variant and bs > 16:
+        pytest.skip("variant 3 changes the min kernel at bs 8/16 only")
     rng = np.random.default_rng(300 + bs)
     for name, g in graph_cases:
         A, (rp, ci, tw) = packed(gbo, g, bs, chunk, variant)
