@@ -184,9 +184,16 @@ __global__ __launch_bounds__(256) void k_spmv(const SpmvArgs a) {
             const int pc = popcw(wd[u]);
             int incl = pc;
             if (a.w) {
-              for (int off = 1; off < BS; off <<= 1) {
-                const int o = __shfl_up(incl, off, BS);
-                if (r >= off) incl += o;
+              if (!__ballot(pc > 1)) {
+                // at most one set bit per row (the common case at bs 32/64):
+                // count the earlier rows of this lane group that have one
+                const uint64_t grp_rows = low_mask<BS>() << (g * BS);
+                incl += __popcll(__ballot(pc != 0) & grp_rows & ((1ull << lane) - 1ull));
+              } else {
+                for (int off = 1; off < BS; off <<= 1) {
+                  const int o = __shfl_up(incl, off, BS);
+                  if (r >= off) incl += o;
+                }
               }
             }
             rank[u] = incl - 1;  // set bits are taken from the last column back
