@@ -17,6 +17,7 @@
 // workgroup leaves, and the host reports an error.  The launch (state reset
 // kernel + persistent kernel) is captured once in a hipGraph and replayed.
 #include <algorithm>
+#include <cstdlib>
 #include <numeric>
 #include <vector>
 
@@ -783,6 +784,16 @@ static int batch_kernel_launch(const BatchArgs& a, hipStream_t st) {
   return GBO_OK;
 }
 
+// Barrier wait limit in ms: kBatchSpinMs, or GBO_BATCH_SPIN_MS from the
+// environment (0 makes every wait expire at once: the error path can be
+// exercised without anything hanging).
+static int64_t batch_spin_ms() {
+  const char* e = std::getenv("GBO_BATCH_SPIN_MS");
+  if (!e || !*e) return kBatchSpinMs;
+  const long long v = std::atoll(e);
+  return v < 0 ? 0 : (int64_t)v;
+}
+
 static int batch_launch(gbo_batch* b, int32_t* d_level, int32_t* d_label, hipStream_t st) {
   BatchArgs a{};
   a.rowptr = b->A.rowptr;
@@ -809,7 +820,7 @@ static int batch_launch(gbo_batch* b, int32_t* d_level, int32_t* d_label, hipStr
   int dev = 0, khz = 0;
   GBO_HIP_CHECK(hipGetDevice(&dev));
   GBO_HIP_CHECK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
-  a.spin_ticks = (int64_t)std::max(khz, 1) * kBatchSpinMs;
+  a.spin_ticks = (int64_t)std::max(khz, 1) * batch_spin_ms();
   a.join_ticks = (int64_t)std::max(khz, 1) * kBatchJoinUs / 1000;
   k_batch_reset<<<1, 1, 0, st>>>(b->d_sync);
   int rc = GBO_OK;
@@ -830,8 +841,8 @@ static int batch_check(gbo_batch* b) {
   unsigned int abort = 0;
   GBO_HIP_CHECK(hipMemcpy(&abort, &b->d_sync->abort, sizeof(abort), hipMemcpyDeviceToHost));
   if (abort)
-    return set_error(GBO_EDEVICE, "gbo_batch_bfs_cc: grid barrier wait exceeded %d ms (workgroups not co-resident)",
-                     kBatchSpinMs);
+    return set_error(GBO_EDEVICE, "gbo_batch_bfs_cc: grid barrier wait exceeded %lld ms",
+                     (long long)batch_spin_ms());
   return GBO_OK;
 }
 

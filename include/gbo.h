@@ -335,8 +335,9 @@ int gbo_autotune(const int32_t* d_rowptr, const int32_t* d_colind, int32_t nrows
  * components assume symmetric graphs.  Outputs are concatenated per graph
  * (offsets from gbo_batch_offsets): d_level as gbo_bfs, d_label = smallest
  * graph-local vertex id of the component, identical to per-graph gbo_bfs /
- * gbo_cc runs.  A grid barrier wait beyond 5 s (workgroups that never got
- * to run) ends the launch and returns GBO_EDEVICE.  No reference
+ * gbo_cc runs.  A grid barrier wait beyond 5 s (GBO_BATCH_SPIN_MS in the
+ * environment overrides the limit) ends the launch and returns GBO_EDEVICE.
+ * No reference
  * counterpart (the reference runs one graph per process). */
 typedef struct gbo_batch gbo_batch;
 int gbo_batch_create(int bs, int32_t ngraphs, const int32_t* h_nrows, const int32_t* const* h_rowptr,

@@ -286,3 +286,22 @@ def test_graph_batch(gbo, po, bs):
         lv, _ = gbo.bfs(A, int(src[i]))
         lb, _ = gbo.cc(A)
         assert np.array_equal(lv, levels[i]) and np.array_equal(lb, labels[i])
+
+
+def test_graph_batch_barrier_timeout(gbo, po, monkeypatch):
+    """A zero barrier wait limit (set from the host) makes every grid barrier
+    wait expire at once: all workgroups leave, the call reports GBO_EDEVICE,
+    and the next call with the normal limit is exact again."""
+    graphs = gbo.gen_er_batch(count=6, n_lo=50, n_hi=800, d_lo=1.0, d_hi=4.0, seed=11)
+    src = np.zeros(len(graphs), np.int32)
+    B = gbo.GraphBatch(graphs, 8)
+    monkeypatch.setenv("GBO_BATCH_SPIN_MS", "0")
+    with pytest.raises(gbo.GboError, match="GBO_EDEVICE"):
+        B.run(src, use_graph=False)
+    monkeypatch.delenv("GBO_BATCH_SPIN_MS")
+    B.run(src, use_graph=False)
+    levels, labels = B.results()
+    for i, g in enumerate(graphs):
+        assert np.array_equal(levels[i], po.bfs(g.nrows, g.rowptr, g.colind, 0)), i
+        assert np.array_equal(labels[i], po.cc(g.nrows, g.rowptr, g.colind)), i
+    B.free()
