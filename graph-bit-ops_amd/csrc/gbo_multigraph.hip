@@ -241,14 +241,6 @@ __device__ __forceinline__ uint32_t wave_or(uint32_t v) {
   return v;
 }
 
-__device__ __forceinline__ int32_t wave_min(int32_t v) {
-  for (int off = kWave / 2; off >= 1; off >>= 1) {
-    const int32_t o = __shfl_xor(v, off);
-    v = o < v ? o : v;
-  }
-  return v;
-}
-
 __device__ __forceinline__ int64_t readlane64(int64_t v, int j) {
   const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, j);
   const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)((uint64_t)v >> 32), j);
