@@ -377,6 +377,9 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched BFS / graph batch / PPR lines")
     ap.add_argument("--profile", action="store_true",
                     help="short run for rocprofv3: only the count sweeps and the copy kernel")
+    ap.add_argument("--profile-sweeps", action="store_true",
+                    help="short run for rocprofv3: BFS full steps and sum / min / min-plus sweeps at the "
+                         "default plan of each --bs, --steps times each")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -396,6 +399,24 @@ def main():
     timer = Timer(gbo)
     copy_gbs = copy_ceiling(gbo, timer)
     log(f"[bench] copy-kernel ceiling {copy_gbs} GB/s")
+    if args.profile_sweeps:
+        # BFS full steps and value sweeps only, each block size's default plan
+        w = gbo.edge_weights(g, 1, 64, 1)
+        out = {}
+        for bs in [int(b) for b in args.bs.split(",")]:
+            A = gbo.BsrMatrix.from_device_csr(d_rp, d_ci, n, n, bs, 0)
+            dxf = gbo.DeviceBuffer.from_numpy((np.random.default_rng(0).random(n) < 0.5).astype(np.float32))
+            dx = gbo.DeviceBuffer((A.nblockcols * bs) // 8)
+            gbo._check(gbo.lib().gbo_pack_vec(bs, dxf.ptr, dx.ptr, n, None))
+            step_ms = time_bfs_step(gbo, A, dx, args.steps, args.warmup, timer)
+            out[bs] = {"variant": A.s.plan_variant, "chunk": A.s.plan_chunk, "bfs_full_step_ms": round(step_ms, 4),
+                       "bfs_full_step_bytes": sweep_bytes(A, n) - 4 * n + 3 * ((n + 7) // 8),
+                       "value_sweeps": time_value_sweeps(gbo, A, g, w, args.steps, args.warmup, timer)}
+            dx.free()
+            dxf.free()
+            A.free()
+        print(json.dumps({"profile_sweeps": out}), flush=True)
+        return
     if args.profile:
         args.no_algos = args.no_cpu = args.no_batch = True
     x = (np.random.default_rng(0).random(n) < 0.5).astype(np.float32)
@@ -415,7 +436,7 @@ def main():
         # autotune: chunk size and (bs 8/16) kernel variant, best time kept
         tried = {}
         variants = [int(v) for v in args.variants.split(",")] if bs <= 16 else [0]
-        if bs == 16:  # variant 3 is bs 8 only (bs 16 runs variant 2 for it)
+        if bs == 16:  # bs 16 count under variant 3 runs variant 0
             variants = [v for v in variants if v != 3]
         for variant in variants:
             for chunk in ((0,) if args.profile else (0, 2 * gbo_default_chunk(bs), gbo_default_chunk(bs) // 2)):
