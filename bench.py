@@ -383,7 +383,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(the bit-BSR sweep would shard by tile-balanced block-row ranges; see DESIGN.md).",
+              "(the bit-BSR sweep would shard by tile-balanced block-row ranges, one range per GPU).",
               file=sys.stderr)
         sys.exit(2)
 
