@@ -25,12 +25,14 @@
 #include "gbo_ops.hpp"
 
 namespace gbo {
-// Grid barrier state and the rotating "anything changed" counters.
+// Grid barrier state and the rotating "anything changed" counters.  The
+// arrival counter, the generation the waiters poll and the abort flag sit
+// on cache lines of their own, so polling does not contend with arrivals.
 struct GridSync {
-  unsigned int arrive;
-  unsigned int gen;
-  unsigned int abort;
-  unsigned int cnt[3];
+  alignas(256) unsigned int arrive;
+  alignas(256) unsigned int gen;
+  alignas(256) unsigned int abort;
+  alignas(256) unsigned int cnt[3];
   unsigned int join;     // tickets taken (| closed bit)
   unsigned int members;  // member count, published at the close
 };
@@ -174,19 +176,28 @@ struct GridBarrier {
   bool leader;
   unsigned int gen = 0, ph = 0;
 
+  // The arrival publishes this workgroup's writes with a release fence and a
+  // relaxed increment; the last arriver acquires them (fence) and releases
+  // the next generation; waiters poll with relaxed loads (an acquiring load
+  // in the loop would invalidate caches on every poll) and acquire once the
+  // generation has moved.  With acquiring polls, the members that finished
+  // a phase early kept invalidating the caches of the ones still working:
+  // every phase of the batch ran slower (512 ER graphs: 21 ms -> 13 ms at
+  // bs 8 with relaxed polls), and a bare barrier cost 73 us instead of 43.
   __device__ bool sync() {
     __shared__ int s_ok;
     __syncthreads();
     if (threadIdx.x == 0) {
       __threadfence();
-      const unsigned int prev = __hip_atomic_fetch_add(&s->arrive, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+      const unsigned int prev = __hip_atomic_fetch_add(&s->arrive, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       if (prev == nparts - 1) {
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         __hip_atomic_store(&s->arrive, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __hip_atomic_fetch_add(&s->gen, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
       } else {
         const int64_t t0 = wall_clock64();
-        while (load_agent(&s->gen) == gen) {
-          if (load_agent(&s->abort)) break;
+        while (__hip_atomic_load(&s->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+          if (__hip_atomic_load(&s->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
           if (wall_clock64() - t0 > spin_ticks) {
             set_abort(s);
             break;
@@ -194,8 +205,8 @@ struct GridBarrier {
           __builtin_amdgcn_s_sleep(2);
         }
       }
-      __threadfence();
-      s_ok = load_agent(&s->abort) == 0;
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      s_ok = __hip_atomic_load(&s->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0;
     }
     ++gen;
     __syncthreads();
