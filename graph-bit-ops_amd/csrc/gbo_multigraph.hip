@@ -115,10 +115,6 @@ __global__ void k_batch_reset(GridSync* s) {
   s->members = 0;
 }
 
-__device__ __forceinline__ unsigned int load_agent(const unsigned int* p) {
-  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-}
-
 __device__ __forceinline__ void set_abort(GridSync* s) {
   __hip_atomic_store(&s->abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -137,14 +133,16 @@ __device__ bool grid_join(GridSync* s, int64_t join_ticks, int64_t spin_ticks, i
       p = (int)t;
       if (t == 0) {
         const int64_t t0 = wall_clock64();
-        while (load_agent(&s->join) < gridDim.x && wall_clock64() - t0 < join_ticks) __builtin_amdgcn_s_sleep(2);
+        while (__hip_atomic_load(&s->join, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < gridDim.x &&
+               wall_clock64() - t0 < join_ticks)
+          __builtin_amdgcn_s_sleep(2);
         const unsigned int n =
             __hip_atomic_fetch_or(&s->join, kJoinClosed, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) & ~kJoinClosed;
         __hip_atomic_store(&s->members, n, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         np = (int)n;
       } else {
         const int64_t t0 = wall_clock64();
-        while ((np = (int)load_agent(&s->members)) == 0) {
+        while ((np = (int)__hip_atomic_load(&s->members, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) == 0) {
           if (wall_clock64() - t0 > spin_ticks) {
             set_abort(s);
             p = -1;
@@ -152,6 +150,7 @@ __device__ bool grid_join(GridSync* s, int64_t join_ticks, int64_t spin_ticks, i
           }
           __builtin_amdgcn_s_sleep(2);
         }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
       }
     }
     s_part = p;
@@ -228,7 +227,8 @@ struct GridBarrier {
     if (threadIdx.x == 0 && *s_any)
       __hip_atomic_fetch_add(&s->cnt[ph % 3], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (!sync()) return -1;
-    if (threadIdx.x == 0) s_total = load_agent(&s->cnt[ph % 3]) != 0;
+    // the barrier's acquire fence already ordered this read after every add
+    if (threadIdx.x == 0) s_total = __hip_atomic_load(&s->cnt[ph % 3], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
     ++ph;
     __syncthreads();
     return s_total;
