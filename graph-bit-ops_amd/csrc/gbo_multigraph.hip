@@ -56,7 +56,6 @@ struct gbo_batch {
   void* d_vis = nullptr;
   int32_t* d_lab = nullptr;    // FastSV parent (union ids)
   int32_t* d_gf = nullptr;
-  int32_t* d_mngf = nullptr;
   int32_t* d_act = nullptr;    // per graph: BFS level, then FastSV iteration, of its last change (2 x ngraphs)
   gbo::GridSync* d_sync = nullptr;
   // captured launch
@@ -84,7 +83,6 @@ struct BatchArgs {
   void* vis;
   int32_t* lab;
   int32_t* gf;
-  int32_t* mngf;
   int32_t* bfs_act;  // last BFS level at which the graph found vertices
   int32_t* cc_act;   // last FastSV iteration that changed the graph
   int32_t* level;
@@ -379,8 +377,24 @@ __device__ __forceinline__ uint64_t bfs_row_rl(const BatchArgs& a, const word_t<
 
 constexpr int kListCap = 8 * kWave;  // set-bit list entries per wave and pass
 
-// mngf[v] = min over the neighbours j of v of gf[j], for the vertices of the
-// block rows R0 .. R0+nrow-1 whose bit is set in amask, bs 8/16.  The rows'
+// FastSV hooking of vertex u (a graph in the current iteration) given m, the
+// minimum of gf over u's neighbours: lab[lab[u]] and lab[u] drop to m, and
+// lab[u] to gf[u] as well.  The sweeps that compute m read only gf, which no
+// hook writes, so each vertex hooks as soon as its m is known.  Labels only
+// decrease, so a stale read of lab[u] is never below its current value and a
+// candidate not below it would leave it unchanged.  A padding vertex past its
+// graph's last one has no neighbours and lab = gf = itself, so its hook
+// changes nothing.
+__device__ __forceinline__ void cc_hook(const BatchArgs& a, int64_t u, int32_t m) {
+  const int32_t fu = a.lab[u];
+  if (m < fu) atomicMin(&a.lab[fu], m);
+  const int32_t gu = a.gf[u];
+  const int32_t cand = m < gu ? m : gu;
+  if (cand < fu) atomicMin(&a.lab[u], cand);
+}
+
+// Hooks each vertex v of the block rows R0 .. R0+nrow-1 whose bit is set in
+// amask with the minimum over the neighbours j of v of gf[j], bs 8/16.  The rows'
 // tiles are contiguous and are taken lane per tile, 64*NS at a time, across
 // row boundaries.  A wave prefix sum of the tiles' popcounts places every set
 // bit (vertex, neighbour) in a wave-private LDS list, so the gathers of gf are
@@ -478,11 +492,11 @@ __device__ __forceinline__ void cc_group_lpt(const BatchArgs& a, int64_t R0, int
   }
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   for (int i = lane; i < nrow * BS; i += kWave)
-    if ((amask >> (i / BS)) & 1) a.mngf[R0 * BS + i] = s_acc[i];
+    if ((amask >> (i / BS)) & 1) cc_hook(a, R0 * BS + i, s_acc[i]);
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
 }
 
-// mngf for one block row, bs 32/64: row per lane, four tiles per group per
+// The same for one block row, bs 32/64: row per lane, four tiles per group per
 // round, one set bit of each per pass (the pass's gathers issued together).
 template <int BS>
 __device__ __forceinline__ void cc_row_rl(const BatchArgs& a, int64_t R, int32_t rs, int32_t re, int lane) {
@@ -520,7 +534,7 @@ __device__ __forceinline__ void cc_row_rl(const BatchArgs& a, int64_t R, int32_t
     const int32_t o = __shfl_xor(acc, off);
     acc = o < acc ? o : acc;
   }
-  if (grp == 0) a.mngf[R * BS + r] = acc;
+  if (grp == 0) cc_hook(a, R * BS + r, acc);
 }
 
 template <int BS>
@@ -690,19 +704,6 @@ __global__ __launch_bounds__(kBatchThreads) void k_batch(const BatchArgs a) {
       }
     }
     if (!bar.sync()) return;
-    for (int64_t u = gt; u < nu; u += nt) {  // hooking + shortcut
-      const int g = a.rowg[u / BS];
-      if (a.cc_act[g] < it || u - a.vbase[g] >= a.n[g]) continue;
-      // labels only decrease, so a stale read of lab[u] is never below its
-      // value and a candidate not below it would leave it unchanged
-      const int32_t m = a.mngf[u];
-      const int32_t fu = a.lab[u];
-      if (m < fu) atomicMin(&a.lab[fu], m);
-      const int32_t gu = a.gf[u];
-      const int32_t cand = m < gu ? m : gu;
-      if (cand < fu) atomicMin(&a.lab[u], cand);
-    }
-    if (!bar.sync()) return;
     bar.open(&s_any);
     for (int64_t u = gt; u < nu; u += nt) {  // grandparents
       const int g = a.rowg[u / BS];
@@ -757,7 +758,7 @@ static void batch_release(gbo_batch* b) {
   if (b->graph) (void)hipGraphDestroy(b->graph);
   gbo_bsr_free(&b->A);
   for (void* p : {(void*)b->d_rowg, (void*)b->d_vbase, (void*)b->d_n, (void*)b->d_out, (void*)b->d_src, b->d_f0,
-                  b->d_f1, b->d_vis, (void*)b->d_lab, (void*)b->d_gf, (void*)b->d_mngf, (void*)b->d_act,
+                  b->d_f1, b->d_vis, (void*)b->d_lab, (void*)b->d_gf, (void*)b->d_act,
                   (void*)b->d_sync})
     if (p) (void)hipFree(p);
   delete b;
@@ -814,7 +815,6 @@ static int batch_launch(gbo_batch* b, int32_t* d_level, int32_t* d_label, hipStr
   a.vis = b->d_vis;
   a.lab = b->d_lab;
   a.gf = b->d_gf;
-  a.mngf = b->d_mngf;
   a.bfs_act = b->d_act;
   a.cc_act = b->d_act + b->ngraphs;
   a.level = d_level;
@@ -923,7 +923,6 @@ int gbo_batch_create(int bs, int32_t ngraphs, const int32_t* h_nrows, const int3
   e = e ? e : hipMalloc(&b->d_vis, wb);
   e = e ? e : hipMalloc((void**)&b->d_lab, sizeof(int32_t) * nu);
   e = e ? e : hipMalloc((void**)&b->d_gf, sizeof(int32_t) * nu);
-  e = e ? e : hipMalloc((void**)&b->d_mngf, sizeof(int32_t) * nu);
   e = e ? e : hipMalloc((void**)&b->d_act, sizeof(int32_t) * 2 * ngraphs);
   e = e ? e : hipMalloc((void**)&b->d_sync, sizeof(GridSync));
   e = e ? e : hipMalloc((void**)&b->d_src, sizeof(int32_t) * ngraphs);
