@@ -265,14 +265,21 @@ __global__ __launch_bounds__(256) void k_pr_scale(const float* __restrict__ p, c
   }
 }
 
-// Per-block partial sums of |a-b| in double; the partials are added in
-// block order by k_sum_partials, so the result is reproducible.
-__global__ __launch_bounds__(256) void k_l1_partial(const float* __restrict__ a, const float* __restrict__ b, int32_t n,
-                                                    double* __restrict__ part) {
+// Per-block partial sums of |q-p| in double, and the next iteration's
+// x[j] = alpha * q[j] / outdeg[j] (as k_pr_scale) from the same read of q.
+// The partials are added in a fixed order by k_sum_partials, so the L1
+// change is reproducible.
+__global__ __launch_bounds__(256) void k_pr_tail(const float* __restrict__ q, const float* __restrict__ p,
+                                                 const int32_t* __restrict__ outdeg, float alpha,
+                                                 float* __restrict__ x, int32_t n, double* __restrict__ part) {
   __shared__ double sh[256];
   double s = 0.0;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    s += fabs((double)a[i] - (double)b[i]);
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const float qi = q[i];
+    s += fabs((double)qi - (double)p[i]);
+    const int32_t d = outdeg[i];
+    x[i] = d > 0 ? alpha * qi / (float)d : 0.0f;
+  }
   sh[threadIdx.x] = s;
   __syncthreads();
   for (int off = 128; off > 0; off >>= 1) {
@@ -282,12 +289,20 @@ __global__ __launch_bounds__(256) void k_l1_partial(const float* __restrict__ a,
   if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
 }
 
-__global__ void k_sum_partials(const double* __restrict__ part, int np, double* out) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    double s = 0.0;
-    for (int i = 0; i < np; ++i) s += part[i];
-    *out = s;
+// *out = the sum of part[0..np) in a fixed order: thread t adds entries
+// t, t+256, ... in turn, then a fixed tree over the threads (one block of
+// 256 threads).
+__global__ __launch_bounds__(256) void k_sum_partials(const double* __restrict__ part, int np, double* out) {
+  __shared__ double sh[256];
+  double s = 0.0;
+  for (int i = threadIdx.x; i < np; i += 256) s += part[i];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
+    __syncthreads();
   }
+  if (threadIdx.x == 0) *out = sh[0];
 }
 
 // ------------------------------------------------------------------ CC
@@ -509,15 +524,17 @@ int gbo_pagerank(const gbo_bsr* AT, const int32_t* d_outdeg, float alpha, double
   double delta = 0.0;
   float* p = d_p;
   float* q = y;  // next iterate
-  while (rc == GBO_OK && it < max_iter) {
+  if (rc == GBO_OK) {
     k_pr_scale<<<blocks_n, 256, 0, st>>>(p, d_outdeg, alpha, x, n);
     GBO_LAUNCH_CHECK();
+  }
+  while (rc == GBO_OK && it < max_iter) {
     a.xval = x;
     a.out = q;
     rc = spmv_launch(AT->bs, OP_SUM, a, st);
     if (rc) break;
-    k_l1_partial<<<lblocks, 256, 0, st>>>(q, p, n, dpart);
-    k_sum_partials<<<1, 64, 0, st>>>(dpart, (int)lblocks, dpart + nparts);
+    k_pr_tail<<<lblocks, 256, 0, st>>>(q, p, d_outdeg, alpha, x, n, dpart);  // x for the next iteration
+    k_sum_partials<<<1, 256, 0, st>>>(dpart, (int)lblocks, dpart + nparts);
     GBO_LAUNCH_CHECK();
     GBO_HIP_CHECK(hipMemcpyAsync(&delta, dpart + nparts, sizeof(double), hipMemcpyDeviceToHost, st));
     GBO_HIP_CHECK(hipStreamSynchronize(st));

@@ -13,6 +13,8 @@
 // min-plus: the result is order independent) or, for the FP32 sum, written
 // as partials and added in chunk order by a fix-up kernel so the result is
 // bitwise reproducible.
+#include <type_traits>
+
 #include "gbo_device.hpp"
 #include "gbo_ops.hpp"
 
@@ -1013,20 +1015,27 @@ __global__ __launch_bounds__(256) void k_count8(const SpmvArgs a) {
 // lane's row word.  Each gathered candidate is folded into its vertex with
 // an LDS atomic min (signed values as x ^ 0x80000000, an order-preserving
 // map to unsigned); rows wholly inside the chunk are stored, split rows
-// combined with global atomics after the prefill, as in k_spmv.
+// combined with global atomics after the prefill, as in k_spmv.  The FP32
+// sum folds the same way into FP64 accumulators with LDS atomic adds: the
+// accumulators are wave-private, so no other wave interleaves its adds and
+// the one wave's adds run in program order, each instruction's same-address
+// lanes serialised by the LDS in a fixed order; split rows leave FP64
+// partials to k_sum_fixup, as in k_spmv.
 constexpr int kValCap = 4 * kWave;  // list entries per wave and pass
 
 template <int BS, int OP>
 __global__ __launch_bounds__(256) void k_value_lpt(const SpmvArgs a) {
   static_assert(BS == 8 || BS == 16, "lane-per-tile tiles of 8 or 32 bytes");
-  static_assert(OP == OP_MIN || OP == OP_MINPLUS, "min / min-plus only");
+  static_assert(OP == OP_MIN || OP == OP_MINPLUS || OP == OP_SUM, "min / min-plus / sum only");
   constexpr bool PLUS = OP == OP_MINPLUS;
+  constexpr bool SUM = OP == OP_SUM;
+  using Acc = typename std::conditional<SUM, double, uint32_t>::type;
   constexpr int NWD = BS == 8 ? 1 : 4;  // 64-bit words per tile
   constexpr uint32_t kFlip = 0x80000000u;
   __shared__ uint32_t s_idx[4][kValCap];
   __shared__ uint32_t s_widx[4][PLUS ? kValCap : 1];
   __shared__ uint16_t s_dst[4][kValCap];
-  __shared__ uint32_t s_acc[4][kWave * BS];
+  __shared__ Acc s_acc[4][kWave * BS];
   const int lane = lane_id();
   const int wv = threadIdx.x / kWave;
   const int c = wave_index();
@@ -1040,7 +1049,7 @@ __global__ __launch_bounds__(256) void k_value_lpt(const SpmvArgs a) {
   uint32_t* idx = s_idx[wv];
   uint32_t* widx = s_widx[wv];
   uint16_t* dst = s_dst[wv];
-  uint32_t* acc = s_acc[wv];
+  Acc* acc = s_acc[wv];
 
   for (int32_t R0 = a.plan[c]; R0 < a.nbr;) {
     const int32_t Rl = R0 + lane;
@@ -1056,7 +1065,7 @@ __global__ __launch_bounds__(256) void k_value_lpt(const SpmvArgs a) {
     const int32_t relx = belong ? rel : INT32_MAX;
     const int64_t w0 = max((int64_t)__builtin_amdgcn_readlane(rsl, 0), cb);
     const int64_t w1 = min((int64_t)__builtin_amdgcn_readlane(rel, nb - 1), ce);
-    for (int k = lane; k < kWave * BS; k += kWave) acc[k] = 0xFFFFFFFFu;
+    for (int k = lane; k < kWave * BS; k += kWave) acc[k] = SUM ? (Acc)0 : (Acc)0xFFFFFFFFu;
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     for (int64_t t0 = w0; t0 < w1; t0 += kWave) {
       const int64_t t = t0 + lane;
@@ -1133,17 +1142,19 @@ __global__ __launch_bounds__(256) void k_value_lpt(const SpmvArgs a) {
         }
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         const int nl = tot - base < kValCap ? tot - base : kValCap;
-        uint32_t v[kValCap / kWave];
+        Acc v[kValCap / kWave];
         int d[kValCap / kWave];
 #pragma unroll
         for (int q = 0; q < kValCap / kWave; ++q) {
           const int e2 = q * kWave + lane;
-          v[q] = 0xFFFFFFFFu;
+          v[q] = SUM ? (Acc)0 : (Acc)0xFFFFFFFFu;
           d[q] = 0;
           if (e2 < nl) {
             d[q] = dst[e2];
             const uint32_t x = xv[idx[e2]];
-            if constexpr (PLUS) {
+            if constexpr (SUM) {
+              v[q] = (double)__uint_as_float(x);
+            } else if constexpr (PLUS) {
               const uint32_t wt = weighted ? a.w[widx[e2]] : 1u;
               v[q] = (x > kInfU32 - wt) ? kInfU32 : x + wt;
             } else {
@@ -1152,8 +1163,12 @@ __global__ __launch_bounds__(256) void k_value_lpt(const SpmvArgs a) {
           }
         }
 #pragma unroll
-        for (int q = 0; q < kValCap / kWave; ++q)
-          if (q * kWave + lane < nl) atomicMin(&acc[d[q]], v[q]);
+        for (int q = 0; q < kValCap / kWave; ++q) {
+          if (q * kWave + lane < nl) {
+            if constexpr (SUM) atomicAdd(&acc[d[q]], v[q]);
+            else atomicMin(&acc[d[q]], v[q]);
+          }
+        }
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
       }
     }
@@ -1166,8 +1181,11 @@ __global__ __launch_bounds__(256) void k_value_lpt(const SpmvArgs a) {
       const int64_t i = (int64_t)(R0 + j) * BS + (p % BS);
       if (p < nb * BS && re > rs && i < a.nrows) {
         const bool full = (rs >= cb) && (re <= ce);
-        const uint32_t m = acc[p];
-        if constexpr (PLUS) {
+        const Acc m = acc[p];
+        if constexpr (SUM) {
+          if (full) ((float*)a.out)[i] = (float)((double)a.scale * m + (double)a.bias);
+          else a.partial[((int64_t)c * 2 + (rs < cb ? 0 : 1)) * BS + (p % BS)] = m;
+        } else if constexpr (PLUS) {
           uint32_t* out = (uint32_t*)a.out;
           if (full) {
             const uint32_t dd = xv[i];
@@ -1223,8 +1241,20 @@ __global__ __launch_bounds__(256) void k_sum_fixup(const SpmvArgs a) {
   const int32_t R = a.plan[c + 1];
   const int64_t rs = a.rowptr[R], re = a.rowptr[R + 1];
   if (!(rs < nb) || rs < c * a.chunk) return;  // not split here, or not the owner
+  // chunks c+1 .. ce-1 hold the rest of the row (a hub row spans hundreds):
+  // their partials are loaded 16 at a time, so the chain costs one memory
+  // latency per 16 chunks, and added in chunk order
+  const int64_t ce = min((int64_t)a.nchunks, (re + a.chunk - 1) / a.chunk);
   double s = a.partial[(c * 2 + 1) * BS + r];
-  for (int64_t c2 = c + 1; c2 < a.nchunks && c2 * a.chunk < re; ++c2) s += a.partial[(c2 * 2 + 0) * BS + r];
+  int64_t c2 = c + 1;
+  for (; c2 + 16 <= ce; c2 += 16) {
+    double v[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) v[u] = a.partial[((c2 + u) * 2 + 0) * BS + r];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) s += v[u];
+  }
+  for (; c2 < ce; ++c2) s += a.partial[(c2 * 2 + 0) * BS + r];
   const int64_t i = (int64_t)R * BS + r;
   if (i < a.nrows) ((float*)a.out)[i] = (float)((double)a.scale * s + (double)a.bias);
 }
@@ -1305,7 +1335,7 @@ static int spmv_launch_t(const SpmvArgs& a, hipStream_t st) {
   } else if constexpr (BS <= 16 && OP == OP_BFS) {
     if (lpt) k_spmv_lpt<BS, OP><<<(unsigned)blocks, 256, 0, st>>>(a);
     else k_spmv<BS, OP><<<(unsigned)blocks, 256, 0, st>>>(a);
-  } else if constexpr (BS <= 16 && (OP == OP_MIN || OP == OP_MINPLUS)) {
+  } else if constexpr (BS <= 16 && (OP == OP_MIN || OP == OP_MINPLUS || OP == OP_SUM)) {
     // variant 3 (the bs 8/16 default): the set-bit list kernel
     if (a.variant == 3 && aligned) k_value_lpt<BS, OP><<<(unsigned)blocks, 256, 0, st>>>(a);
     else k_spmv<BS, OP><<<(unsigned)blocks, 256, 0, st>>>(a);

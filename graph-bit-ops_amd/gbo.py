@@ -516,50 +516,68 @@ def bsrbmm_sum(A_row: BsrMatrix, B_col: BsrMatrix) -> int:
 
 
 # ----------------------------------------------------------------- drivers
-def bfs(AT: BsrMatrix, source: int = 0, max_levels: int = 4096, A: BsrMatrix | None = None):
+def _result(dbuf: DeviceBuffer, dtype, n: int, out: DeviceBuffer | None):
+    """The result as a host array, or the caller's device buffer as it is."""
+    return out if out is not None else dbuf.to_numpy(dtype, n)
+
+
+def _out_buffer(out: DeviceBuffer | None, nbytes: int) -> DeviceBuffer:
+    if out is None:
+        return DeviceBuffer(max(nbytes, 4))
+    if out.nbytes < nbytes:
+        raise ValueError(f"output buffer of {out.nbytes} bytes, {nbytes} needed")
+    return out
+
+
+def bfs(AT: BsrMatrix, source: int = 0, max_levels: int = 4096, A: BsrMatrix | None = None,
+        out: DeviceBuffer | None = None):
     """(levels int32[n], per-level discovered counts).  With A (the out-edge
     matrix; AT itself for an undirected graph) small frontiers are pushed
-    (gbo_bfs_do); without it every level is pulled over AT (gbo_bfs)."""
-    dl = DeviceBuffer(4 * max(AT.nrows, 1))
+    (gbo_bfs_do); without it every level is pulled over AT (gbo_bfs).  With
+    out (a device buffer of >= 4n bytes) the levels stay there and out is
+    returned in place of the host array."""
+    dl = _out_buffer(out, 4 * AT.nrows)
     sizes = np.zeros(max_levels, np.int64)
     nl = c_int32()
     if A is None:
         _check(lib().gbo_bfs(AT.ref, source, dl.ptr, _ptr(sizes), max_levels, byref(nl), None))
     else:
         _check(lib().gbo_bfs_do(AT.ref, A.ref, source, dl.ptr, _ptr(sizes), max_levels, byref(nl), None))
-    return dl.to_numpy(np.int32, AT.nrows), sizes[: min(nl.value, max_levels)].copy()
+    return _result(dl, np.int32, AT.nrows, out), sizes[: min(nl.value, max_levels)].copy()
 
 
-def sssp(AT: BsrMatrix, source: int = 0, weights: TileValues | None = None):
-    """(uint32 distances, iterations); INF = 0xFFFFFFFF."""
-    dd = DeviceBuffer(4 * max(AT.nrows, 1))
+def sssp(AT: BsrMatrix, source: int = 0, weights: TileValues | None = None, out: DeviceBuffer | None = None):
+    """(uint32 distances, iterations); INF = 0xFFFFFFFF.  out as for bfs."""
+    dd = _out_buffer(out, 4 * AT.nrows)
     it = c_int32()
     w = weights.tval.ptr if weights else None
     wo = weights.woff.ptr if weights else None
     _check(lib().gbo_sssp(AT.ref, w, wo, source, dd.ptr, byref(it), None))
-    return dd.to_numpy(np.uint32, AT.nrows), it.value
+    return _result(dd, np.uint32, AT.nrows, out), it.value
 
 
-def pagerank(AT: BsrMatrix, outdeg: np.ndarray, alpha: float = 0.85, tol: float = 1e-7, max_iter: int = 1000,
-             strict: bool = False):
+def pagerank(AT: BsrMatrix, outdeg, alpha: float = 0.85, tol: float = 1e-7, max_iter: int = 1000,
+             strict: bool = False, out: DeviceBuffer | None = None):
     """(float32 scores, iterations, last L1 change).  Reaching max_iter first
-    (GBO_ENOCONV) raises only when strict=True; the iterate is returned."""
-    do = DeviceBuffer.from_numpy(np.asarray(outdeg, np.int32))
-    dp = DeviceBuffer(4 * max(AT.nrows, 1))
+    (GBO_ENOCONV) raises only when strict=True; the iterate is returned.
+    outdeg is a host array or a device buffer of int32; out as for bfs."""
+    do = _dev(outdeg, np.int32)
+    dp = _out_buffer(out, 4 * AT.nrows)
     it = c_int32()
     delta = c_double()
     rc = lib().gbo_pagerank(AT.ref, do.ptr, alpha, tol, max_iter, dp.ptr, byref(it), byref(delta), None)
     if rc != GBO_OK and (strict or rc != -7):
         _check(rc)
-    return dp.to_numpy(np.float32, AT.nrows), it.value, delta.value
+    return _result(dp, np.float32, AT.nrows, out), it.value, delta.value
 
 
-def cc(A: BsrMatrix):
-    """(int32 labels = smallest vertex id of the component, iterations)."""
-    dl = DeviceBuffer(4 * max(A.nrows, 1))
+def cc(A: BsrMatrix, out: DeviceBuffer | None = None):
+    """(int32 labels = smallest vertex id of the component, iterations).  out
+    as for bfs."""
+    dl = _out_buffer(out, 4 * A.nrows)
     it = c_int32()
     _check(lib().gbo_cc(A.ref, dl.ptr, byref(it), None))
-    return dl.to_numpy(np.int32, A.nrows), it.value
+    return _result(dl, np.int32, A.nrows, out), it.value
 
 
 def tc(csr: Csr, bs: int) -> int:
