@@ -186,10 +186,13 @@ int gbo_bsr_plan(gbo_bsr* A, int32_t chunk_tiles, gbo_stream_t stream);
  * step is dense (that count kernel gives each row's number of frontier
  * neighbours, a pack kernel keeps the nonzero unvisited rows; every tile is
  * read, so gbo_bfs takes it while fewer than half the vertices are visited
- * and variant 1 after); at bs 8 and 16, min / min-plus list the tiles' set
- * bits in LDS so the value gathers spread over all lanes (bs 16 COUNT and
+ * and variant 1 after); at bs 8 and 16, min / min-plus / FP32 sum list the
+ * tiles' set bits in LDS so the value gathers spread over all lanes (the sum
+ * folds them into FP64 accumulators private to one wavefront; bs 16 COUNT and
  * BFS run variant 0).
- * Results are identical; only speed differs (the autotuner picks).
+ * Integer results are identical; only speed differs (the autotuner picks).
+ * The FP32 sum's order of additions depends on the chunk size and variant,
+ * each fixed by the plan, so its last bits may differ between plans.
  * gbo_bsr_create / gbo_bsr_create_dev set variant 3 for bs 8/16, 0 otherwise. */
 int gbo_bsr_plan_ex(gbo_bsr* A, int32_t chunk_tiles, int32_t variant, gbo_stream_t stream);
 void gbo_bsr_plan_free(gbo_bsr* A);
