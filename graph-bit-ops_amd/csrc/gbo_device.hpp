@@ -46,6 +46,21 @@ __device__ __forceinline__ uint64_t wave_or64(uint64_t v) {
   return v;
 }
 
+// Inclusive prefix sum of v over the 64 lanes with DPP moves (no LDS round
+// trips): shifts by 1, 2, 4, 8 inside each 16-lane row, then lane 15 of
+// rows 0 and 2 is added to rows 1 and 3 and lane 31 to rows 2 and 3.  A
+// lane whose DPP source falls outside its row (or whose row is masked off)
+// reads the `old` operand, 0.  Every lane of the wave must be active.
+__device__ __forceinline__ int wave_incl_scan(int v) {
+  v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xF, 0xF, false);  // row_shr:1
+  v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xF, 0xF, false);  // row_shr:2
+  v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xF, 0xF, false);  // row_shr:4
+  v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xF, 0xF, false);  // row_shr:8
+  v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xA, 0xF, false);  // row_bcast:15 -> rows 1, 3
+  v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xC, 0xF, false);  // row_bcast:31 -> rows 2, 3
+  return v;
+}
+
 // *flag = 1 if any lane of the wave passes `any`.  Termination flags are
 // only ever set to the same value, so this is a plain store after a plain
 // read instead of a read-modify-write: tens of thousands of waves do not

@@ -193,12 +193,7 @@ __global__ __launch_bounds__(256) void k_ppr_spmm8(const PprArgs A) {
         const uint64_t word = lane < nt ? tw[b + lane] : 0ull;
         const uint32_t vb = lane < nt ? (uint32_t)A.colind[b + lane] * BS + (BS - 1) : 0u;
         const int cnt = __popcll(word);
-        int incl = cnt;  // inclusive prefix sum of the popcounts over the lanes
-#pragma unroll
-        for (int off = 1; off < kWave; off <<= 1) {
-          const int o = __shfl_up(incl, off);
-          if (lane >= off) incl += o;
-        }
+        const int incl = wave_incl_scan(cnt);  // inclusive prefix sum of the popcounts over the lanes
         const int excl = incl - cnt;
         const int tot = __builtin_amdgcn_readlane(incl, kWave - 1);
         for (int base = 0; base < tot; base += kPprCap) {
