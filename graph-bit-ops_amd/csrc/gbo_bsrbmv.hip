@@ -1093,12 +1093,22 @@ __global__ __launch_bounds__(256) void k_value_lpt(const SpmvArgs a) {
           for (int q = 0; q < NWD; ++q) w[q] = 0;
         }
       }
-      // window row of tile t (all lanes take part in the shuffles)
+      // window row of tile t = the number of window rows that end at or
+      // before it (relx is nondecreasing over the lanes).  Usually few rows
+      // end inside the round: the rows ended before it are one ballot, and
+      // each row end inside it one lane read; with more than 8, a 6-step
+      // shuffle search (all lanes take part).
       const int64_t tc = t < w1 ? t : w1 - 1;
+      const uint64_t inner = __ballot((int64_t)relx > t0 && (int64_t)relx < t0 + kWave);
       int j = 0;
+      if (__popcll(inner) <= 8) {
+        j = __popcll(__ballot((int64_t)relx <= t0));
+        for (uint64_t m = inner; m; m &= m - 1) j += tc >= (int64_t)__builtin_amdgcn_readlane(relx, __builtin_ctzll(m));
+      } else {
 #pragma unroll
-      for (int step = kWave / 2; step >= 1; step >>= 1)
-        if ((int64_t)__shfl(relx, j + step - 1) <= tc) j += step;
+        for (int step = kWave / 2; step >= 1; step >>= 1)
+          if ((int64_t)__shfl(relx, j + step - 1) <= tc) j += step;
+      }
       const int jr = j * BS;
       int cnt = 0;
 #pragma unroll

@@ -441,7 +441,12 @@ __device__ __forceinline__ void cc_group_lpt(const BatchArgs& a, int64_t R0, int
 #pragma unroll
       for (int q = 0; q < NWD; ++q) cnt += __popcll(w[u][q]);
     }
-    const int incl = wave_incl_scan(cnt);  // inclusive prefix sum of the popcounts over the lanes
+    int incl = cnt;  // inclusive prefix sum of the popcounts over the lanes
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+      const int o = __shfl_up(incl, off);
+      if (lane >= off) incl += o;
+    }
     const int excl = incl - cnt;
     const int tot = __builtin_amdgcn_readlane(incl, kWave - 1);
     for (int base = 0; base < tot; base += kListCap) {
