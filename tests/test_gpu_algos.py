@@ -123,6 +123,35 @@ def test_pagerank_closed_forms(gbo):
     np.testing.assert_allclose(p, 1.0 / 40, rtol=1e-5)
 
 
+@pytest.mark.parametrize("bs", (8, 32))
+def test_driver_results_in_device_memory(gbo, po, bs):
+    """out= leaves a driver's result in the caller's device buffer (the
+    oracle's answer, and the same bits as the host-array return); PageRank
+    takes its out-degrees from device memory too.  A buffer too small for
+    n results is refused before any launch."""
+    g = gbo.gen_rmat(12, 16, seed=3)
+    n = g.nrows
+    A = gbo.BsrMatrix.from_csr(g, bs)
+    d = gbo.DeviceBuffer(4 * n)
+    lv = po.bfs(n, g.rowptr, g.colind, 0)
+    r, _ = gbo.bfs(A, 0, out=d)
+    assert r is d
+    assert np.array_equal(d.to_numpy(np.int32, n), lv)
+    gbo.bfs(A, 0, A=A, out=d)
+    assert np.array_equal(d.to_numpy(np.int32, n), lv)
+    gbo.sssp(A, 0, out=d)
+    assert np.array_equal(d.to_numpy(np.uint32, n), np.where(lv < 0, gbo.INF_U32, lv).astype(np.uint32))
+    gbo.cc(A, out=d)
+    assert np.array_equal(d.to_numpy(np.int32, n), po.cc(n, g.rowptr, g.colind))
+    d_od = gbo.DeviceBuffer.from_numpy(g.outdeg().astype(np.int32))
+    _, it, _ = gbo.pagerank(A, d_od, 0.85, 1e-8, 300, out=d)
+    p_host, it_host, _ = gbo.pagerank(A, g.outdeg(), 0.85, 1e-8, 300)
+    assert it == it_host
+    assert np.array_equal(d.to_numpy(np.float32, n).view(np.uint32), p_host.view(np.uint32))
+    with pytest.raises(ValueError):
+        gbo.cc(A, out=gbo.DeviceBuffer(4 * n - 4))
+
+
 @pytest.mark.parametrize("bs", BS)
 def test_cc(gbo, po, graph_cases, bs):
     for name, g in graph_cases:
