@@ -176,10 +176,12 @@ __global__ __launch_bounds__(256) void k_spmv(const SpmvArgs a) {
             wo[u] = 0;
             if (tu < te) {
               cc[u] = colind[tu];
-              wd[u] = (uint64_t)ld_value_tile<BS>(&tiles[tu * BS + r]);
-              if (a.w) wo[u] = a.woff[tu];
-              // inactive source block column: the tile is skipped (group-uniform)
-              if (a.active && !((a.active[cc[u] >> 6] >> (cc[u] & 63)) & 1ull)) wd[u] = 0;
+              // inactive source block column: the tile and its weight offset
+              // are not read (group-uniform)
+              if (!a.active || ((a.active[cc[u] >> 6] >> (cc[u] & 63)) & 1ull)) {
+                wd[u] = (uint64_t)ld_value_tile<BS>(&tiles[tu * BS + r]);
+                if (a.w) wo[u] = a.woff[tu];
+              }
             }
             // index of this row's last set bit among the tile's weights:
             // rows r' < r come first in tile bit order
@@ -1074,23 +1076,22 @@ __global__ __launch_bounds__(256) void k_value_lpt(const SpmvArgs a) {
       for (int q = 0; q < NWD; ++q) w[q] = 0;
       uint32_t cv = 0, wb = 0;
       if (t < w1) {
-        if constexpr (BS == 8) {
-          w[0] = ((const uint64_t*)a.tiles)[t];
-        } else {  // word q holds rows 4q..4q+3 as 16-bit fields
-          const uint4* tp = (const uint4*)a.tiles + 2 * t;
-          const uint4 lo = tp[0], hi = tp[1];
-          w[0] = lo.x | ((uint64_t)lo.y << 32);
-          w[1] = lo.z | ((uint64_t)lo.w << 32);
-          w[2] = hi.x | ((uint64_t)hi.y << 32);
-          w[3] = hi.z | ((uint64_t)hi.w << 32);
-        }
         const int32_t col = colind[t];
         cv = (uint32_t)col * BS;
-        if (weighted) wb = (uint32_t)a.woff[t];
-        // inactive source block column: the tile is skipped
-        if (PLUS && a.active && !((a.active[col >> 6] >> (col & 63)) & 1ull)) {
-#pragma unroll
-          for (int q = 0; q < NWD; ++q) w[q] = 0;
+        // inactive source block column (min-plus): the tile and its weight
+        // offset are not read
+        if (!(PLUS && a.active) || ((a.active[col >> 6] >> (col & 63)) & 1ull)) {
+          if constexpr (BS == 8) {
+            w[0] = ((const uint64_t*)a.tiles)[t];
+          } else {  // word q holds rows 4q..4q+3 as 16-bit fields
+            const uint4* tp = (const uint4*)a.tiles + 2 * t;
+            const uint4 lo = tp[0], hi = tp[1];
+            w[0] = lo.x | ((uint64_t)lo.y << 32);
+            w[1] = lo.z | ((uint64_t)lo.w << 32);
+            w[2] = hi.x | ((uint64_t)hi.y << 32);
+            w[3] = hi.z | ((uint64_t)hi.w << 32);
+          }
+          if (weighted) wb = (uint32_t)a.woff[t];
         }
       }
       // window row of tile t = the number of window rows that end at or
