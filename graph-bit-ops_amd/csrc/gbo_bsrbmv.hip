@@ -20,6 +20,21 @@
 
 namespace gbo {
 
+// Tiles per group and round of the row-group min / sum loops: their words
+// and column indices are loaded together, then one set bit of each is
+// gathered per pass.  Eight measured faster than four for min at bs 32/64
+// and the sum at bs 64 (R-MAT s22: 3-6 %), slower for the sum at bs 32.
+template <int BS, int OP>
+constexpr int kValueTiles = (BS == 64 || (BS == 32 && OP == OP_MIN)) ? 8 : 4;
+
+template <int NU>
+__device__ __forceinline__ bool any_word(const uint64_t* wd) {
+  uint64_t o = 0;
+#pragma unroll
+  for (int u = 0; u < NU; ++u) o |= wd[u];
+  return o != 0;
+}
+
 // Identity written for a block row with no tiles (lane-per-row, see k_spmv).
 template <int BS, int OP>
 __device__ __forceinline__ void write_empty_row(const SpmvArgs& a, int32_t R) {
@@ -244,19 +259,20 @@ __global__ __launch_bounds__(256) void k_spmv(const SpmvArgs a) {
         // gathers of a pass are independent.
         const float* __restrict__ x = (const float*)a.xval;
         double acc = 0.0;
-        for (int64_t t = s + g; t < te; t += 4 * G) {
-          int32_t cc[4];
-          uint64_t wd[4];
+        constexpr int NU = kValueTiles<BS, OP>;
+        for (int64_t t = s + g; t < te; t += NU * G) {
+          int32_t cc[NU];
+          uint64_t wd[NU];
 #pragma unroll
-          for (int u = 0; u < 4; ++u) {
+          for (int u = 0; u < NU; ++u) {
             const int64_t tu = t + u * G;
             cc[u] = tu < te ? colind[tu] : 0;
             wd[u] = tu < te ? (uint64_t)ld_value_tile<BS>(&tiles[tu * BS + r]) : 0ull;
           }
-          while (wd[0] | wd[1] | wd[2] | wd[3]) {
-            float v[4];
+          while (any_word<NU>(wd)) {
+            float v[NU];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
+            for (int u = 0; u < NU; ++u) {
               v[u] = 0.0f;
               if (wd[u]) {
                 const int p = __builtin_ctzll(wd[u]);
@@ -265,7 +281,7 @@ __global__ __launch_bounds__(256) void k_spmv(const SpmvArgs a) {
               }
             }
 #pragma unroll
-            for (int u = 0; u < 4; ++u) acc += (double)v[u];
+            for (int u = 0; u < NU; ++u) acc += (double)v[u];
           }
         }
         for (int off = BS; off < kWave; off <<= 1) acc += __shfl_xor(acc, off);
@@ -280,19 +296,20 @@ __global__ __launch_bounds__(256) void k_spmv(const SpmvArgs a) {
       } else if constexpr (OP == OP_MIN) {
         const int32_t* __restrict__ x = (const int32_t*)a.xval;
         int32_t acc = INT32_MAX;
-        for (int64_t t = s + g; t < te; t += 4 * G) {
-          int32_t cc[4];
-          uint64_t wd[4];
+        constexpr int NU = kValueTiles<BS, OP>;
+        for (int64_t t = s + g; t < te; t += NU * G) {
+          int32_t cc[NU];
+          uint64_t wd[NU];
 #pragma unroll
-          for (int u = 0; u < 4; ++u) {
+          for (int u = 0; u < NU; ++u) {
             const int64_t tu = t + u * G;
             cc[u] = tu < te ? colind[tu] : 0;
             wd[u] = tu < te ? (uint64_t)ld_value_tile<BS>(&tiles[tu * BS + r]) : 0ull;
           }
-          while (wd[0] | wd[1] | wd[2] | wd[3]) {
-            int32_t v[4];  // the 4 gathers of a pass are issued before any is used
+          while (any_word<NU>(wd)) {
+            int32_t v[NU];  // the NU gathers of a pass are issued before any is used
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
+            for (int u = 0; u < NU; ++u) {
               v[u] = INT32_MAX;
               if (wd[u]) {
                 const int p = __builtin_ctzll(wd[u]);
@@ -301,7 +318,7 @@ __global__ __launch_bounds__(256) void k_spmv(const SpmvArgs a) {
               }
             }
 #pragma unroll
-            for (int u = 0; u < 4; ++u) acc = v[u] < acc ? v[u] : acc;
+            for (int u = 0; u < NU; ++u) acc = v[u] < acc ? v[u] : acc;
           }
         }
         for (int off = BS; off < kWave; off <<= 1) {
