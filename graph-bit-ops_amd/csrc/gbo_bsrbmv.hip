@@ -146,8 +146,24 @@ __global__ __launch_bounds__(256) void k_spmv(const SpmvArgs a) {
         }
         // Tile words and column indices of eight tiles are loaded together,
         // then their frontier words: two dependent loads per eight tiles.
+        // With a sparse frontier the column indices go first, then the
+        // frontier words, and only the tiles whose frontier word is nonzero
+        // are read (group-uniform).
         int hit = 0;
         int64_t t = s + g;
+        if (a.sparse_frontier) {
+          for (; t < te; t += 8 * G) {
+            int32_t cc[8];
+            W fw[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) cc[u] = t + u * G < te ? colind[t + u * G] : -1;
+#pragma unroll
+            for (int u = 0; u < 8; ++u) fw[u] = cc[u] >= 0 ? fb[cc[u]] : (W)0;
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+              if (fw[u]) hit |= (W)(tiles[(t + u * G) * BS + r] & fw[u]) != 0;
+          }
+        }
         for (; t + 7 * G < te; t += 8 * G) {
           int32_t cc[8];
           W ww[8];
@@ -1317,10 +1333,11 @@ static int spmv_launch_t(const SpmvArgs& a, hipStream_t st) {
   const bool stream = (BS <= 16 && OP == OP_COUNT && a.variant == 2 && aligned);
   // variant 3 at bs 8 is the lean count kernel; its BFS step is the
   // lane-per-tile kernel (faster than the row-group one at bs 8)
-  const bool lpt = (BS <= 16 && (OP == OP_COUNT || OP == OP_BFS) && a.variant == 1) ||
-                   (BS == 8 && OP == OP_BFS && a.variant == 3);
+  const bool lpt = ((BS <= 16 && (OP == OP_COUNT || OP == OP_BFS) && a.variant == 1) ||
+                    (BS == 8 && OP == OP_BFS && a.variant == 3)) &&
+                   !(OP == OP_BFS && a.sparse_frontier);
   if constexpr (BS == 8 && OP == OP_BFS) {
-    if (a.variant == 3 && a.bfs_counts && aligned) {
+    if (a.variant == 3 && a.bfs_counts && aligned && !a.sparse_frontier) {
       // dense step: the lean count kernel gives each row's number of
       // frontier neighbours (every tile is read; no visited-row skip)
       SpmvArgs c = a;

@@ -53,6 +53,8 @@ struct SpmvArgs {
                         // the count kernel into it and packs the rows with a nonzero count (dense step)
   double* partial;      // SUM: split-row partials [nchunks][2][bs]
   const uint64_t* active;  // MINPLUS: optional per-block-column activity bits (nullable)
+  int sparse_frontier;  // BFS: most frontier words are zero, so a tile is read only where its
+                        // block column's frontier word is nonzero (row-group kernel)
   int accumulate;       // COUNT: add into y
   float scale, bias;    // SUM: y = scale * (A x) + bias  (scale 1, bias 0 for the plain op)
 };
