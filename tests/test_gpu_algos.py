@@ -42,6 +42,23 @@ def test_bfs_levels(gbo, po, graph_cases, bs):
             assert np.array_equal(olv, want) and np.array_equal(osz, hist), (name, s)
 
 
+@pytest.mark.parametrize("bs", (8, 16))
+@pytest.mark.parametrize("variant", (0, 1, 3))
+def test_bfs_levels_variants(gbo, po, graph_cases, bs, variant):
+    """Every bs 8/16 BFS step kernel (row-group, lane-per-tile, and at bs 8
+    the dense step) gives the oracle's levels; at bs 16 the levels with a
+    sparse frontier take the row-group kernel's path that reads only the
+    tiles under nonzero frontier words.  Chunk 3 splits most rows."""
+    for name, g in graph_cases:
+        for chunk in (0, 3):
+            A = gbo.BsrMatrix.from_csr(g, bs, 0, plan=False)
+            A.plan(chunk, variant)
+            for s in sources(g):
+                lv, _ = gbo.bfs(A, s)
+                assert np.array_equal(lv, po.bfs(g.nrows, g.rowptr, g.colind, s)), (name, chunk, s)
+            A.free()
+
+
 @pytest.mark.parametrize("bs", BS)
 def test_bfs_direction_optimizing(gbo, po, graph_cases, bs):
     """gbo_bfs_do (push for frontiers of at most n/64 vertices, pull over AT
