@@ -280,7 +280,7 @@ __global__ __launch_bounds__(256) void k_ppr_fixup(const PprArgs A) {
 // Deterministic column-wise reduction over a [n][ld] column block of
 // `cols` <= 64 columns; mode 0: sum |Y - P|, mode 1: sum L * (Y - P),
 // mode 2: sum |Y|.  Each block writes 64 partials; k_colred_final adds
-// them in block order.
+// them in a fixed order.
 constexpr int kColredBlocks = 2048;  // fixed, so the reduction order is the same on every GPU
 
 __global__ __launch_bounds__(256) void k_colred(const float* __restrict__ Y, const float* __restrict__ P,
@@ -314,12 +314,33 @@ __global__ __launch_bounds__(256) void k_colred(const float* __restrict__ Y, con
         sh[threadIdx.x] + sh[threadIdx.x + 64] + sh[threadIdx.x + 128] + sh[threadIdx.x + 192];
 }
 
-__global__ void k_colred_final(const double* __restrict__ part, int nb, double* out) {
-  const int k = threadIdx.x;
-  if (k >= 64) return;
+// One block of 1024 threads: 16 per column, thread (s, k) adds the partials
+// of blocks s, s+16, s+32, ... in order (eight loads in flight), then the
+// 16 sums of a column are added in s order -- a fixed order for a fixed nb.
+constexpr int kColredFinalThreads = 1024;
+__global__ __launch_bounds__(kColredFinalThreads) void k_colred_final(const double* __restrict__ part, int nb,
+                                                                      double* out) {
+  constexpr int NS = kColredFinalThreads / 64;
+  __shared__ double sh[NS][64];
+  const int k = threadIdx.x & 63, sidx = threadIdx.x >> 6;
   double s = 0.0;
-  for (int b = 0; b < nb; ++b) s += part[(int64_t)b * 64 + k];
-  out[k] = s;
+  int b = sidx;
+  for (; b + 7 * NS < nb; b += 8 * NS) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = part[(int64_t)(b + u * NS) * 64 + k];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) s += v[u];
+  }
+  for (; b < nb; b += NS) s += part[(int64_t)b * 64 + k];
+  sh[sidx][k] = s;
+  __syncthreads();
+  if (sidx == 0) {
+    double t = 0.0;
+#pragma unroll
+    for (int q = 0; q < NS; ++q) t += sh[q][k];
+    out[k] = t;
+  }
 }
 
 __global__ void k_invdeg(const int32_t* __restrict__ deg, float* __restrict__ inv, int32_t n) {
@@ -451,7 +472,7 @@ static int colreduce(const float* Y, const float* P, const float* L, int32_t n, 
     const int32_t cols = std::min(64, K - k0);
     k_colred<<<nb, 256, 0, st>>>(Y + k0, P ? P + k0 : nullptr, L ? L + k0 : nullptr, n, K, cols, mode, d_part);
     GBO_LAUNCH_CHECK();
-    k_colred_final<<<1, 64, 0, st>>>(d_part, nb, d_out);
+    k_colred_final<<<1, kColredFinalThreads, 0, st>>>(d_part, nb, d_out);
     GBO_LAUNCH_CHECK();
     GBO_HIP_CHECK(hipMemcpyAsync(h_out + k0, d_out, sizeof(double) * cols, hipMemcpyDeviceToHost, st));
   }
