@@ -74,16 +74,41 @@ __global__ __launch_bounds__(256) void k_msbfs(const MsArgs a) {
       uint64_t acc[KW];
 #pragma unroll
       for (int w = 0; w < KW; ++w) acc[w] = 0;
-      for (int64_t t = s + g; t < te; t += G) {
-        const int32_t col = a.colind[t];
-        if (!((a.active[col >> 6] >> (col & 63)) & 1ull)) continue;  // group-uniform
-        uint64_t wd = (uint64_t)tiles[t * BS + r];
-        while (wd) {
-          const int p = __builtin_ctzll(wd);
-          wd &= wd - 1;
-          const int64_t v = (int64_t)col * BS + (BS - 1 - p);
+      // Four tiles per group and round: their column indices, then their
+      // activity bits, then the words of the active ones (inactive block
+      // columns hold no frontier bit: their tiles are not read), then one
+      // set bit of each tile per pass with its KW frontier words, so the
+      // loads of a stage are in flight together.
+      constexpr int U = 4;
+      for (int64_t t = s + g; t < te; t += U * G) {
+        int32_t cc[U];
+        uint64_t wd[U];
 #pragma unroll
-          for (int w = 0; w < KW; ++w) acc[w] |= a.F[v * KW + w];
+        for (int u = 0; u < U; ++u) cc[u] = t + u * G < te ? a.colind[t + u * G] : -1;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          wd[u] = 0;
+          if (cc[u] >= 0 && ((a.active[cc[u] >> 6] >> (cc[u] & 63)) & 1ull))  // group-uniform
+            wd[u] = (uint64_t)tiles[(t + u * G) * BS + r];
+        }
+        while (wd[0] | wd[1] | wd[2] | wd[3]) {
+          uint64_t f[U][KW];
+#pragma unroll
+          for (int u = 0; u < U; ++u) {
+#pragma unroll
+            for (int w = 0; w < KW; ++w) f[u][w] = 0;
+            if (wd[u]) {
+              const int p = __builtin_ctzll(wd[u]);
+              wd[u] &= wd[u] - 1;
+              const int64_t v = (int64_t)cc[u] * BS + (BS - 1 - p);
+#pragma unroll
+              for (int w = 0; w < KW; ++w) f[u][w] = a.F[v * KW + w];
+            }
+          }
+#pragma unroll
+          for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int w = 0; w < KW; ++w) acc[w] |= f[u][w];
         }
       }
 #pragma unroll
