@@ -387,7 +387,7 @@ constexpr int kListCap = 8 * kWave;  // set-bit list entries per wave and pass
 // changes nothing.
 __device__ __forceinline__ void cc_hook(const BatchArgs& a, int64_t u, int32_t m) {
   const int32_t fu = a.lab[u];
-  if (m < fu) atomicMin(&a.lab[fu], m);
+  if (m < fu && m < a.lab[fu]) atomicMin(&a.lab[fu], m);  // skip atomics that cannot lower it
   const int32_t gu = a.gf[u];
   const int32_t cand = m < gu ? m : gu;
   if (cand < fu) atomicMin(&a.lab[u], cand);
