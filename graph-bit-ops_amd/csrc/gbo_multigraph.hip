@@ -634,7 +634,7 @@ __global__ __launch_bounds__(kBatchThreads) void k_batch(const BatchArgs a) {
         for (uint64_t mk = pend; mk;) {  // each graph's marker once
           const int gl = __builtin_amdgcn_readlane(l_g, __builtin_ctzll(mk));
           mk &= ~__ballot(word != 0 && l_g == gl);
-          if (lane == 0 && gl != marked) atomicMax(&a.bfs_act[gl], k + 1);
+          if (lane == 0 && gl != marked && a.bfs_act[gl] < k + 1) atomicMax(&a.bfs_act[gl], k + 1);
           marked = gl;
         }
         for (int p = lane; p < nrow * BS; p += kWave) {  // levels of the new vertices
@@ -658,7 +658,7 @@ __global__ __launch_bounds__(kBatchThreads) void k_batch(const BatchArgs a) {
           const int g = __builtin_amdgcn_readlane(l_g, j);
           if (lane == 0) {
             vis[R] = (W)(vw | word);
-            if (g != marked) atomicMax(&a.bfs_act[g], k + 1);
+            if (g != marked && a.bfs_act[g] < k + 1) atomicMax(&a.bfs_act[g], k + 1);
             s_any = 1;
           }
           marked = g;
@@ -721,7 +721,7 @@ __global__ __launch_bounds__(kBatchThreads) void k_batch(const BatchArgs a) {
       while (pend) {
         const int gl = __builtin_amdgcn_readlane(g, __builtin_ctzll(pend));
         pend &= ~__ballot(changed && g == gl);
-        if (lane == 0) atomicMax(&a.cc_act[gl], it + 1);
+        if (lane == 0 && a.cc_act[gl] < it + 1) atomicMax(&a.cc_act[gl], it + 1);
       }
     }
     const int more = bar.close(&s_any);
