@@ -96,11 +96,28 @@ __device__ __forceinline__ uint64_t atomic_or_word(word_t<BS>* base, int64_t idx
 
 // Reductions over the lanes of one group of BS lanes, or across the G
 // groups that hold the same row index (xor offsets BS, 2BS, ...).
+// 32-bit values reduce inside a 16-lane row with DPP moves (no LDS round
+// trip): quad_perm swaps of 1 and 2, then row_half_mirror (lane i <-> 7-i of
+// each 8) and row_mirror (i <-> 15-i), which pair every lane with the other
+// half once its quad / half already holds the minimum.  Every lane of the
+// wave must be active.
 template <int BS, class T>
 __device__ __forceinline__ T group_min(T v) {
-  for (int off = 1; off < BS; off <<= 1) {
-    T o = __shfl_xor(v, off);
-    v = o < v ? o : v;
+  if constexpr (sizeof(T) == 4) {
+    auto step = [&](T o) { v = o < v ? o : v; };
+    step((T)__builtin_amdgcn_update_dpp((int)0, (int)v, 0xB1, 0xF, 0xF, false));  // quad_perm [1,0,3,2]
+    if constexpr (BS > 2) step((T)__builtin_amdgcn_update_dpp((int)0, (int)v, 0x4E, 0xF, 0xF, false));  // [2,3,0,1]
+    if constexpr (BS > 4) step((T)__builtin_amdgcn_update_dpp((int)0, (int)v, 0x141, 0xF, 0xF, false));  // row_half_mirror
+    if constexpr (BS > 8) step((T)__builtin_amdgcn_update_dpp((int)0, (int)v, 0x140, 0xF, 0xF, false));  // row_mirror
+    for (int off = 16; off < BS; off <<= 1) {
+      T o = __shfl_xor(v, off);
+      v = o < v ? o : v;
+    }
+  } else {
+    for (int off = 1; off < BS; off <<= 1) {
+      T o = __shfl_xor(v, off);
+      v = o < v ? o : v;
+    }
   }
   return v;
 }
