@@ -565,7 +565,7 @@ __global__ __launch_bounds__(256) void k_spmv_lpt(const SpmvArgs a) {
         }
 #pragma unroll
         for (int k = 0; k < NW; ++k)
-          for (int off = 1; off < kWave; off <<= 1) hit[k] |= (uint64_t)__shfl_xor((long long)hit[k], off);
+          hit[k] = wave_or64(hit[k]);
         // row r hit <=> its byte (bs 8) / short (bs 16) of hit is non-zero
         uint64_t word = 0;
 #pragma unroll

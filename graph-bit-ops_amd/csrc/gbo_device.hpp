@@ -40,10 +40,19 @@ __device__ __forceinline__ uint64_t brev_low(uint64_t x) {
   return __brevll(x) >> (64 - BS);
 }
 
-// OR of v over the wave (every lane gets it).
+// OR of v over the wave (every lane gets it, as a wave-uniform value): DPP
+// rotations by 1, 2, 4, 8 inside each 16-lane row, then the four rows'
+// results read from one lane each.  Every lane of the wave must be active.
+__device__ __forceinline__ uint32_t wave_or32(uint32_t v) {
+  v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x121, 0xF, 0xF, false);  // row_ror:1
+  v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x122, 0xF, 0xF, false);  // row_ror:2
+  v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x124, 0xF, 0xF, false);  // row_ror:4
+  v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x128, 0xF, 0xF, false);  // row_ror:8
+  return (uint32_t)(__builtin_amdgcn_readlane((int)v, 0) | __builtin_amdgcn_readlane((int)v, 16) |
+                    __builtin_amdgcn_readlane((int)v, 32) | __builtin_amdgcn_readlane((int)v, 48));
+}
 __device__ __forceinline__ uint64_t wave_or64(uint64_t v) {
-  for (int off = kWave / 2; off >= 1; off >>= 1) v |= __shfl_xor(v, off);
-  return v;
+  return (uint64_t)wave_or32((uint32_t)v) | ((uint64_t)wave_or32((uint32_t)(v >> 32)) << 32);
 }
 
 // Inclusive prefix sum of v over the 64 lanes with DPP moves (no LDS round
