@@ -178,9 +178,8 @@ __global__ __launch_bounds__(256) void k_spmv(const SpmvArgs a) {
           for (int u = 0; u < 8; ++u) hit |= (W)(ww[u] & fb[cc[u]]) != 0;
         }
         for (; t < te; t += G) hit |= (W)(tiles[t * BS + r] & fb[colind[t]]) != 0;
-        for (int off = BS; off < kWave; off <<= 1) hit |= __shfl_xor(hit, off);
-        const uint64_t bal = __ballot(hit);
-        const uint64_t word = brev_low<BS>(bal & low_mask<BS>()) & ~(uint64_t)vword & low_mask<BS>();
+        const uint64_t bal = fold_groups<BS>(__ballot(hit));  // row r: any of its lane groups hit
+        const uint64_t word = brev_low<BS>(bal) & ~(uint64_t)vword & low_mask<BS>();
         if (lane == 0) {
           if (full) {
             nx[R] = (W)word;

@@ -78,6 +78,17 @@ __device__ __forceinline__ void wave_flag(bool any, unsigned long long* flag) {
   if (__ballot(any) && (threadIdx.x & (kWave - 1)) == 0 && *flag == 0ull) *flag = 1ull;
 }
 
+// OR of the 64/BS groups of BS bits of a ballot: bit r is set when any lane
+// with lane % BS == r had its bit set (a row's hit over the lane groups that
+// hold it), by scalar shifts instead of a cross-lane reduction.
+template <int BS>
+__device__ __forceinline__ uint64_t fold_groups(uint64_t b) {
+  if constexpr (BS <= 32) b |= b >> 32;
+  if constexpr (BS <= 16) b |= b >> 16;
+  if constexpr (BS <= 8) b |= b >> 8;
+  return BS == 64 ? b : (b & ((1ull << BS) - 1ull));
+}
+
 // Mask with the low BS bits set.
 template <int BS>
 __device__ __forceinline__ uint64_t low_mask() {

@@ -371,8 +371,7 @@ __device__ __forceinline__ uint64_t bfs_row_rl(const BatchArgs& a, const word_t<
       for (int u = 0; u < D; ++u) hit |= (W)(ww[u] & ff[u]) != 0;
     }
   }
-  for (int off = BS; off < kWave; off <<= 1) hit |= __shfl_xor(hit, off);
-  return __ballot(hit) & low_mask<BS>();
+  return fold_groups<BS>(__ballot(hit));  // row r: any of its lane groups hit
 }
 
 constexpr int kListCap = 8 * kWave;  // set-bit list entries per wave and pass
