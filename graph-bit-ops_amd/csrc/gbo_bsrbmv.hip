@@ -1182,31 +1182,39 @@ __global__ __launch_bounds__(256) void k_value_lpt(const SpmvArgs a) {
         }
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         const int nl = tot - base < kValCap ? tot - base : kValCap;
-        Acc v[kValCap / kWave];
-        int d[kValCap / kWave];
+        // All gathers of the pass are issued before any is used: a slot
+        // holding list entries (nl is wave-uniform) loads in every lane,
+        // lanes past the list reading entry 0 (a pass has nl >= 1); their
+        // values are left out when folding.  (Loads and uses inside one
+        // per-lane branch made the compiler wait for each gather in turn.)
+        constexpr int NQ = kValCap / kWave;
+        uint32_t xr[NQ], wr[NQ];
+        int d[NQ];
 #pragma unroll
-        for (int q = 0; q < kValCap / kWave; ++q) {
-          const int e2 = q * kWave + lane;
-          v[q] = SUM ? (Acc)0 : (Acc)0xFFFFFFFFu;
+        for (int q = 0; q < NQ; ++q) {
+          xr[q] = 0;
+          wr[q] = 1u;
           d[q] = 0;
-          if (e2 < nl) {
-            d[q] = dst[e2];
-            const uint32_t x = xv[idx[e2]];
-            if constexpr (SUM) {
-              v[q] = (double)__uint_as_float(x);
-            } else if constexpr (PLUS) {
-              const uint32_t wt = weighted ? a.w[widx[e2]] : 1u;
-              v[q] = (x > kInfU32 - wt) ? kInfU32 : x + wt;
-            } else {
-              v[q] = x ^ kFlip;
+          if (q * kWave < nl) {  // wave-uniform
+            const int e2 = q * kWave + lane;
+            const int ee = e2 < nl ? e2 : 0;
+            d[q] = dst[ee];
+            xr[q] = xv[idx[ee]];
+            if constexpr (PLUS) {
+              if (weighted) wr[q] = a.w[widx[ee]];
             }
           }
         }
 #pragma unroll
-        for (int q = 0; q < kValCap / kWave; ++q) {
+        for (int q = 0; q < NQ; ++q) {
           if (q * kWave + lane < nl) {
-            if constexpr (SUM) atomicAdd(&acc[d[q]], v[q]);
-            else atomicMin(&acc[d[q]], v[q]);
+            if constexpr (SUM) {
+              atomicAdd(&acc[d[q]], (double)__uint_as_float(xr[q]));
+            } else if constexpr (PLUS) {
+              atomicMin(&acc[d[q]], (xr[q] > kInfU32 - wr[q]) ? kInfU32 : xr[q] + wr[q]);
+            } else {
+              atomicMin(&acc[d[q]], xr[q] ^ kFlip);
+            }
           }
         }
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
