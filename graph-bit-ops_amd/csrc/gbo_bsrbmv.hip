@@ -200,21 +200,30 @@ __global__ __launch_bounds__(256) void k_spmv(const SpmvArgs a) {
           uint64_t wd[4];
           int rank[4];
           int64_t wo[4];  // first weight of each tile
+          // three stages, each over the four tiles so that their loads are in
+          // flight together: column indices (slots past the row load its
+          // first tile, s < te), activity bits, then the words and weight
+          // offsets of the active tiles -- an inactive source block column's
+          // tile is not read (group-uniform)
+          bool act[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) cc[u] = colind[t + u * G < te ? t + u * G : s];
 #pragma unroll
           for (int u = 0; u < 4; ++u) {
-            const int64_t tu = t + u * G;
-            cc[u] = 0;
+            const uint64_t aw = a.active ? a.active[cc[u] >> 6] : ~0ull;  // cc[u] is a valid column
+            act[u] = t + u * G < te && ((aw >> (cc[u] & 63)) & 1ull);
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
             wd[u] = 0;
             wo[u] = 0;
-            if (tu < te) {
-              cc[u] = colind[tu];
-              // inactive source block column: the tile and its weight offset
-              // are not read (group-uniform)
-              if (!a.active || ((a.active[cc[u] >> 6] >> (cc[u] & 63)) & 1ull)) {
-                wd[u] = (uint64_t)ld_value_tile<BS>(&tiles[tu * BS + r]);
-                if (a.w) wo[u] = a.woff[tu];
-              }
+            if (act[u]) {
+              wd[u] = (uint64_t)ld_value_tile<BS>(&tiles[(t + u * G) * BS + r]);
+              if (a.w) wo[u] = a.woff[t + u * G];
             }
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
             // index of this row's last set bit among the tile's weights:
             // rows r' < r come first in tile bit order
             const int pc = popcw(wd[u]);
@@ -282,9 +291,14 @@ __global__ __launch_bounds__(256) void k_spmv(const SpmvArgs a) {
           uint64_t wd[NU];
 #pragma unroll
           for (int u = 0; u < NU; ++u) {
+            // slots past the row load its first tile (s < te) and are then
+            // emptied: the loads are unconditional, so all of them are in
+            // flight together
             const int64_t tu = t + u * G;
-            cc[u] = tu < te ? colind[tu] : 0;
-            wd[u] = tu < te ? (uint64_t)ld_value_tile<BS>(&tiles[tu * BS + r]) : 0ull;
+            const int64_t tc = tu < te ? tu : s;
+            cc[u] = colind[tc];
+            const uint64_t wv = (uint64_t)ld_value_tile<BS>(&tiles[tc * BS + r]);
+            wd[u] = tu < te ? wv : 0ull;
           }
           while (any_word<NU>(wd)) {
             float v[NU];
@@ -319,9 +333,14 @@ __global__ __launch_bounds__(256) void k_spmv(const SpmvArgs a) {
           uint64_t wd[NU];
 #pragma unroll
           for (int u = 0; u < NU; ++u) {
+            // slots past the row load its first tile (s < te) and are then
+            // emptied: the loads are unconditional, so all of them are in
+            // flight together
             const int64_t tu = t + u * G;
-            cc[u] = tu < te ? colind[tu] : 0;
-            wd[u] = tu < te ? (uint64_t)ld_value_tile<BS>(&tiles[tu * BS + r]) : 0ull;
+            const int64_t tc = tu < te ? tu : s;
+            cc[u] = colind[tc];
+            const uint64_t wv = (uint64_t)ld_value_tile<BS>(&tiles[tc * BS + r]);
+            wd[u] = tu < te ? wv : 0ull;
           }
           while (any_word<NU>(wd)) {
             int32_t v[NU];  // the NU gathers of a pass are issued before any is used
