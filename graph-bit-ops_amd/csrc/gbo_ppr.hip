@@ -213,10 +213,16 @@ __global__ __launch_bounds__(256) void k_ppr_spmm8(const PprArgs A) {
             uint32_t e[NB];
 #pragma unroll
             for (int k = 0; k < NB; ++k) e[k] = __builtin_amdgcn_readfirstlane(list[min(i + k, nl - 1)]);
+            // the NB row reads are unconditional (e[k] is a valid entry,
+            // lanes past K read column 0) so they are in flight together;
+            // conditional reads were each waited for in turn
+            const int lc = col_ok ? lane : 0;
             float xk[NB];
 #pragma unroll
-            for (int k = 0; k < NB; ++k)
-              xk[k] = (i + k < nl && col_ok) ? A.X[(int64_t)(e[k] & 0x1FFFFFFFu) * A.ld + lane] : 0.0f;
+            for (int k = 0; k < NB; ++k) {
+              const float xv = A.X[(int64_t)(e[k] & 0x1FFFFFFFu) * A.ld + lc];
+              xk[k] = (i + k < nl && col_ok) ? xv : 0.0f;
+            }
 #pragma unroll
             for (int k = 0; k < NB; ++k)
               if (i + k < nl) acc[e[k] >> 29] += (double)xk[k];
