@@ -157,10 +157,15 @@ __global__ __launch_bounds__(256) void k_spmv(const SpmvArgs a) {
           for (; t < te; t += 8 * G) {
             int32_t cc[8];
             W fw[8];
+            // column and frontier-word loads unconditional (slots past the
+            // row read tile s's) so each stage's loads are in flight together
 #pragma unroll
-            for (int u = 0; u < 8; ++u) cc[u] = t + u * G < te ? colind[t + u * G] : -1;
+            for (int u = 0; u < 8; ++u) cc[u] = colind[t + u * G < te ? t + u * G : s];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) fw[u] = cc[u] >= 0 ? fb[cc[u]] : (W)0;
+            for (int u = 0; u < 8; ++u) {
+              const W f = fb[cc[u]];
+              fw[u] = t + u * G < te ? f : (W)0;
+            }
 #pragma unroll
             for (int u = 0; u < 8; ++u)
               if (fw[u]) hit |= (W)(tiles[(t + u * G) * BS + r] & fw[u]) != 0;
