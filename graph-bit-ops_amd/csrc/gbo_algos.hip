@@ -121,26 +121,26 @@ __global__ __launch_bounds__(256) void k_popc_words(const word_t<BS>* __restrict
 }
 
 // Pull BFS over AT; with A (the out-edge matrix, AT itself for an undirected
-// graph) levels whose frontier holds at most n/64 vertices are pushed.
+// graph) levels whose frontier holds at most n/64 vertices are pushed.  The
+// frontier-size slots and the dense step's row counts come from the plan's
+// scratch; the frontier, next and visited words are this call's.
 template <int BS>
 static int bfs_t(const gbo_bsr* A, const gbo_bsr* Aout, int32_t source, int32_t* d_level, int64_t* sizes,
                  int32_t max_levels, int32_t* nlevels, hipStream_t st) {
   using W = word_t<BS>;
   const int32_t nbr = A->nblockrows, n = A->nrows;
-  W *f = nullptr, *nx = nullptr, *vis = nullptr;
-  int64_t* d_cnt = nullptr;
-  GBO_HIP_CHECK(hipMallocAsync((void**)&f, sizeof(W) * (size_t)std::max(nbr, 1) * 3 + 64, st));
-  nx = f + std::max(nbr, 1);
-  vis = nx + std::max(nbr, 1);
-  GBO_HIP_CHECK(hipMallocAsync((void**)&d_cnt, sizeof(int64_t), st));
-  unsigned long long* slots = nullptr;
-  int rc = count_slots_create(&slots, st);
+  AsyncAllocs mem(st);
+  W* f = nullptr;
+  int rc = mem.alloc(&f, sizeof(W) * (size_t)std::max(nbr, 1) * 3 + 64, "gbo_bfs");
   if (rc) return rc;
+  W* nx = f + std::max(nbr, 1);
+  W* vis = nx + std::max(nbr, 1);
   int32_t* segoff = nullptr;
-  if (Aout) GBO_HIP_CHECK(hipMallocAsync((void**)&segoff, sizeof(int32_t) * ((size_t)nbr + 1), st));
-  PlanView pv;
-  rc = acquire_plan(A, &pv, st);
-  if (rc) return rc;
+  if (Aout && (rc = mem.alloc(&segoff, sizeof(int32_t) * ((size_t)nbr + 1), "gbo_bfs_do"))) return rc;
+  PlanGuard g(st);
+  if ((rc = acquire_plan(A, &g.pv, st))) return rc;
+  const PlanView& pv = g.pv;
+  int64_t* d_cnt = pv.count;
   const unsigned blocks_n = (unsigned)((std::max<int64_t>(n, nbr) + 255) / 256);
   k_bfs_init<BS><<<blocks_n, 256, 0, st>>>(f, vis, d_level, nbr, n, source);
   GBO_LAUNCH_CHECK();
@@ -156,12 +156,10 @@ static int bfs_t(const gbo_bsr* A, const gbo_bsr* Aout, int32_t source, int32_t*
   a.chunk = pv.chunk;
   a.nchunks = pv.nchunks;
   a.variant = pv.variant;
-  a.count_slots = slots;
+  a.count_slots = pv.slots;
   // bs 8 variant 3: levels while fewer than half the vertices are visited
   // take the dense step (count kernel + pack); later ones skip visited rows
-  int32_t* dense_counts = nullptr;
-  if (BS == 8 && pv.variant == 3)
-    GBO_HIP_CHECK(hipMallocAsync((void**)&dense_counts, sizeof(int32_t) * 8 * ((size_t)nbr + 1), st));
+  int32_t* dense_counts = (BS == 8 && pv.variant == 3) ? pv.bfs_counts : nullptr;
   int64_t visited = 1;
   int32_t lvl = 0;
   int64_t fsize = 1;  // vertices in the current frontier
@@ -173,8 +171,7 @@ static int bfs_t(const gbo_bsr* A, const gbo_bsr* Aout, int32_t source, int32_t*
       GBO_HIP_CHECK(hipMemsetAsync(nx, 0, sizeof(W) * (size_t)std::max(nbr, 1), st));
       k_push_segs<BS><<<(unsigned)(((int64_t)nbr + 1 + 255) / 256), 256, 0, st>>>(f, Aout->rowptr, nbr, segoff);
       GBO_LAUNCH_CHECK();
-      rc = scan_exclusive<int32_t, int32_t>(segoff, segoff, (int64_t)nbr + 1, st);
-      if (rc) break;
+      if ((rc = scan_exclusive<int32_t, int32_t>(segoff, segoff, (int64_t)nbr + 1, st))) return rc;
       k_push<BS><<<2048, 256, 0, st>>>(f, vis, nx, Aout->rowptr, Aout->colind, (const W*)Aout->tiles, segoff, nbr);
       GBO_LAUNCH_CHECK();
       k_popc_words<BS><<<(unsigned)((nbr + 255) / 256), 256, 0, st>>>(nx, nbr, (unsigned long long*)d_cnt);
@@ -189,10 +186,8 @@ static int bfs_t(const gbo_bsr* A, const gbo_bsr* Aout, int32_t source, int32_t*
       // (bs 16-64: R-MAT s22 pull BFS bs 64 23.6 -> 17.6 ms; at bs 8 the dense
       // count step stays faster than the gated row-group kernel)
       a.sparse_frontier = BS >= 16 && fsize * BS <= (int64_t)n / 4;
-      rc = spmv_launch(BS, OP_BFS, a, st);
-      if (rc) break;
-      rc = count_fold(slots, d_cnt, st);
-      if (rc) break;
+      if ((rc = spmv_launch(BS, OP_BFS, a, st))) return rc;
+      if ((rc = count_fold(pv.slots, d_cnt, st))) return rc;
     }
     int64_t cnt = 0;
     GBO_HIP_CHECK(hipMemcpyAsync(&cnt, d_cnt, sizeof(int64_t), hipMemcpyDeviceToHost, st));
@@ -207,15 +202,7 @@ static int bfs_t(const gbo_bsr* A, const gbo_bsr* Aout, int32_t source, int32_t*
     std::swap(f, nx);
   }
   if (nlevels) *nlevels = lvl + 1;
-  release_plan(&pv, st);
-  W* base = std::min(std::min(f, nx), vis);
-  (void)hipFreeAsync(base, st);
-  (void)hipFreeAsync(d_cnt, st);
-  (void)hipFreeAsync(slots, st);
-  if (dense_counts) (void)hipFreeAsync(dense_counts, st);
-  if (segoff) (void)hipFreeAsync(segoff, st);
-  GBO_HIP_CHECK(hipStreamSynchronize(st));
-  return rc;
+  return GBO_OK;
 }
 
 // ------------------------------------------------------------------ SSSP
@@ -433,16 +420,17 @@ int gbo_sssp(const gbo_bsr* AT, const uint32_t* d_w, const int64_t* d_woff, int3
   hipStream_t st = stream;
   const int bs = AT->bs;
   const int64_t nact = ((int64_t)AT->nblockcols + 63) / 64;
-  uint32_t *fr = nullptr, *cand = nullptr;
-  unsigned long long *active = nullptr, *changed = nullptr;
-  GBO_HIP_CHECK(hipMallocAsync((void**)&fr, sizeof(uint32_t) * (size_t)std::max(n, 1) * 2, st));
-  cand = fr + std::max(n, 1);
-  GBO_HIP_CHECK(hipMallocAsync((void**)&active, sizeof(unsigned long long) * (size_t)(nact + 1) * 2, st));
-  changed = active + 2 * nact + 1;
+  AsyncAllocs mem(st);
+  uint32_t* fr = nullptr;
+  unsigned long long* active = nullptr;
+  if ((rc = mem.alloc(&fr, sizeof(uint32_t) * (size_t)std::max(n, 1) * 2, "gbo_sssp"))) return rc;
+  if ((rc = mem.alloc(&active, sizeof(unsigned long long) * (size_t)(nact + 1) * 2, "gbo_sssp"))) return rc;
+  uint32_t* cand = fr + std::max(n, 1);
+  unsigned long long* changed = active + 2 * nact + 1;
   unsigned long long* active2 = active + nact;
-  PlanView pv;
-  rc = acquire_plan(AT, &pv, st);
-  if (rc) return rc;
+  PlanGuard g(st);
+  if ((rc = acquire_plan(AT, &g.pv, st))) return rc;
+  const PlanView& pv = g.pv;
   GBO_HIP_CHECK(hipMemsetAsync(active, 0, sizeof(unsigned long long) * nact, st));
   const unsigned blocks_n = (unsigned)((n + 255) / 256);
   k_sssp_init<<<blocks_n, 256, 0, st>>>(d_dist, fr, active, n, bs, source);
@@ -467,28 +455,21 @@ int gbo_sssp(const gbo_bsr* AT, const uint32_t* d_w, const int64_t* d_woff, int3
     a.xval = fr;
     a.out = cand;
     a.active = (const uint64_t*)act_cur;
-    rc = spmv_launch(bs, OP_MINPLUS, a, st);
-    if (rc) break;
+    if ((rc = spmv_launch(bs, OP_MINPLUS, a, st))) return rc;
     GBO_HIP_CHECK(hipMemsetAsync(act_next, 0, sizeof(unsigned long long) * nact, st));
     GBO_HIP_CHECK(hipMemsetAsync(changed, 0, sizeof(unsigned long long), st));
     k_sssp_update<<<blocks_n, 256, 0, st>>>(cand, d_dist, fr, act_next, n, bs, changed);
     GBO_LAUNCH_CHECK();
     ++it;
     uint64_t h = 0;
-    rc = read_u64(changed, &h, st);
-    if (rc || h == 0) break;
-    if (it > n + 1) {  // distances only decrease: cannot happen
-      rc = set_error(GBO_ENOCONV, "gbo_sssp: no convergence after %d iterations", it);
-      break;
-    }
+    if ((rc = read_u64(changed, &h, st))) return rc;
+    if (h == 0) break;
+    if (it > n + 1)  // distances only decrease: cannot happen
+      return set_error(GBO_ENOCONV, "gbo_sssp: no convergence after %d iterations", it);
     std::swap(act_cur, act_next);
   }
   if (h_iters) *h_iters = it;
-  release_plan(&pv, st);
-  (void)hipFreeAsync(fr, st);
-  (void)hipFreeAsync(active, st);
-  GBO_HIP_CHECK(hipStreamSynchronize(st));
-  return rc;
+  return GBO_OK;
 }
 
 int gbo_pagerank(const gbo_bsr* AT, const int32_t* d_outdeg, float alpha, double tol, int32_t max_iter, float* d_p,
@@ -500,18 +481,17 @@ int gbo_pagerank(const gbo_bsr* AT, const int32_t* d_outdeg, float alpha, double
   if (!(alpha >= 0.0f && alpha < 1.0f) || max_iter < 1) return set_error(GBO_EINVAL, "gbo_pagerank: bad alpha/max_iter");
   if (n == 0) return GBO_OK;
   hipStream_t st = stream;
-  float *x = nullptr, *y = nullptr;
-  double* partial = nullptr;
-  double* dpart = nullptr;
   const int nparts = 1024;
-  GBO_HIP_CHECK(hipMallocAsync((void**)&x, sizeof(float) * (size_t)n * 2, st));
-  y = x + n;
-  GBO_HIP_CHECK(hipMallocAsync((void**)&dpart, sizeof(double) * (nparts + 1), st));
-  PlanView pv;
-  rc = acquire_plan(AT, &pv, st);
-  if (rc) return rc;
-  GBO_HIP_CHECK(hipMallocAsync((void**)&partial, sizeof(double) * 2 * (size_t)pv.nchunks * AT->bs, st));
-  rc = fill<float>(d_p, n, 1.0f / (float)n, st);
+  AsyncAllocs mem(st);
+  float* x = nullptr;
+  double* dpart = nullptr;
+  if ((rc = mem.alloc(&x, sizeof(float) * (size_t)n * 2, "gbo_pagerank"))) return rc;
+  if ((rc = mem.alloc(&dpart, sizeof(double) * (nparts + 1), "gbo_pagerank"))) return rc;
+  float* y = x + n;
+  PlanGuard g(st);
+  if ((rc = acquire_plan(AT, &g.pv, st))) return rc;
+  const PlanView& pv = g.pv;
+  if ((rc = fill<float>(d_p, n, 1.0f / (float)n, st))) return rc;
   SpmvArgs a{};
   a.rowptr = AT->rowptr;
   a.colind = AT->colind;
@@ -524,7 +504,7 @@ int gbo_pagerank(const gbo_bsr* AT, const int32_t* d_outdeg, float alpha, double
   a.chunk = pv.chunk;
   a.nchunks = pv.nchunks;
   a.variant = pv.variant;
-  a.partial = partial;
+  a.partial = pv.partial;  // split-row partials from the plan's scratch
   a.scale = 1.0f;
   a.bias = (1.0f - alpha) / (float)n;
   const unsigned blocks_n = (unsigned)((n + 255) / 256);
@@ -533,15 +513,12 @@ int gbo_pagerank(const gbo_bsr* AT, const int32_t* d_outdeg, float alpha, double
   double delta = 0.0;
   float* p = d_p;
   float* q = y;  // next iterate
-  if (rc == GBO_OK) {
-    k_pr_scale<<<blocks_n, 256, 0, st>>>(p, d_outdeg, alpha, x, n);
-    GBO_LAUNCH_CHECK();
-  }
-  while (rc == GBO_OK && it < max_iter) {
+  k_pr_scale<<<blocks_n, 256, 0, st>>>(p, d_outdeg, alpha, x, n);
+  GBO_LAUNCH_CHECK();
+  while (it < max_iter) {
     a.xval = x;
     a.out = q;
-    rc = spmv_launch(AT->bs, OP_SUM, a, st);
-    if (rc) break;
+    if ((rc = spmv_launch(AT->bs, OP_SUM, a, st))) return rc;
     k_pr_tail<<<lblocks, 256, 0, st>>>(q, p, d_outdeg, alpha, x, n, dpart);  // x for the next iteration
     k_sum_partials<<<1, 256, 0, st>>>(dpart, (int)lblocks, dpart + nparts);
     GBO_LAUNCH_CHECK();
@@ -551,17 +528,15 @@ int gbo_pagerank(const gbo_bsr* AT, const int32_t* d_outdeg, float alpha, double
     std::swap(p, q);
     if (delta < tol) break;
   }
-  if (rc == GBO_OK && p != d_p) GBO_HIP_CHECK(hipMemcpyAsync(d_p, p, sizeof(float) * n, hipMemcpyDeviceToDevice, st));
-  release_plan(&pv, st);
-  (void)hipFreeAsync(partial, st);
-  (void)hipFreeAsync(x, st);
-  (void)hipFreeAsync(dpart, st);
-  GBO_HIP_CHECK(hipStreamSynchronize(st));
+  if (p != d_p) {
+    GBO_HIP_CHECK(hipMemcpyAsync(d_p, p, sizeof(float) * n, hipMemcpyDeviceToDevice, st));
+    GBO_HIP_CHECK(hipStreamSynchronize(st));
+  }
   if (h_iters) *h_iters = it;
   if (h_last_delta) *h_last_delta = delta;
-  if (rc == GBO_OK && !(delta < tol))
+  if (!(delta < tol))
     return set_error(GBO_ENOCONV, "gbo_pagerank: not converged after %d iterations (delta %.3e)", it, delta);
-  return rc;
+  return GBO_OK;
 }
 
 int gbo_cc(const gbo_bsr* A, int32_t* d_label, int32_t* h_iters, gbo_stream_t stream) {
@@ -571,18 +546,19 @@ int gbo_cc(const gbo_bsr* A, int32_t* d_label, int32_t* h_iters, gbo_stream_t st
   if (!d_label) return set_error(GBO_EINVAL, "gbo_cc: null label");
   if (n == 0) return GBO_OK;
   hipStream_t st = stream;
-  int32_t *gf = nullptr, *mngf = nullptr;
+  AsyncAllocs mem(st);
+  int32_t* gf = nullptr;
   unsigned long long* changed = nullptr;
-  GBO_HIP_CHECK(hipMallocAsync((void**)&gf, sizeof(int32_t) * (size_t)n * 2, st));
-  mngf = gf + n;
-  GBO_HIP_CHECK(hipMallocAsync((void**)&changed, sizeof(unsigned long long), st));
+  if ((rc = mem.alloc(&gf, sizeof(int32_t) * (size_t)n * 2, "gbo_cc"))) return rc;
+  if ((rc = mem.alloc(&changed, sizeof(unsigned long long), "gbo_cc"))) return rc;
+  int32_t* mngf = gf + n;
   int32_t* f = d_label;
   const unsigned blocks_n = (unsigned)((n + 255) / 256);
   k_iota<<<blocks_n, 256, 0, st>>>(f, gf, n);
   GBO_LAUNCH_CHECK();
-  PlanView pv;
-  rc = acquire_plan(A, &pv, st);
-  if (rc) return rc;
+  PlanGuard g(st);
+  if ((rc = acquire_plan(A, &g.pv, st))) return rc;
+  const PlanView& pv = g.pv;
   SpmvArgs a{};
   a.rowptr = A->rowptr;
   a.colind = A->colind;
@@ -599,35 +575,27 @@ int gbo_cc(const gbo_bsr* A, int32_t* d_label, int32_t* h_iters, gbo_stream_t st
   while (true) {
     a.xval = gf;
     a.out = mngf;
-    rc = spmv_launch(A->bs, OP_MIN, a, st);
-    if (rc) break;
+    if ((rc = spmv_launch(A->bs, OP_MIN, a, st))) return rc;
     k_cc_hook<<<blocks_n, 256, 0, st>>>(mngf, gf, f, n);
     GBO_HIP_CHECK(hipMemsetAsync(changed, 0, sizeof(unsigned long long), st));
     k_cc_grand<<<blocks_n, 256, 0, st>>>(f, gf, n, changed);
     GBO_LAUNCH_CHECK();
     ++it;
     uint64_t h = 0;
-    rc = read_u64(changed, &h, st);
-    if (rc || h == 0) break;
-    if (it > n + 2) {
-      rc = set_error(GBO_ENOCONV, "gbo_cc: no convergence");
-      break;
-    }
+    if ((rc = read_u64(changed, &h, st))) return rc;
+    if (h == 0) break;
+    if (it > n + 2) return set_error(GBO_ENOCONV, "gbo_cc: no convergence");
   }
-  while (rc == GBO_OK) {  // make every tree a star
+  while (true) {  // make every tree a star
     GBO_HIP_CHECK(hipMemsetAsync(changed, 0, sizeof(unsigned long long), st));
     k_cc_jump<<<blocks_n, 256, 0, st>>>(f, n, changed);
     GBO_LAUNCH_CHECK();
     uint64_t h = 0;
-    rc = read_u64(changed, &h, st);
+    if ((rc = read_u64(changed, &h, st))) return rc;
     if (h == 0) break;
   }
   if (h_iters) *h_iters = it;
-  release_plan(&pv, st);
-  (void)hipFreeAsync(gf, st);
-  (void)hipFreeAsync(changed, st);
-  GBO_HIP_CHECK(hipStreamSynchronize(st));
-  return rc;
+  return GBO_OK;
 }
 
 int gbo_tc(int bs, int32_t nrows, const int32_t* h_rp, const int32_t* h_ci, uint64_t* h_count, gbo_stream_t stream) {

@@ -196,22 +196,23 @@ static int msbfs_t(const gbo_bsr* A, const int32_t* h_src, int32_t nsrc, int32_t
   const int32_t n = A->nrows;
   const int64_t words = (int64_t)std::max(n, 1) * KW;
   const int64_t nact = ((int64_t)A->nblockcols + 63) / 64 + 1;
+  AsyncAllocs mem(st);
   uint64_t* buf = nullptr;
-  GBO_HIP_CHECK(hipMallocAsync((void**)&buf, sizeof(uint64_t) * (3 * words + 2 * nact + 1), st));
+  int32_t* d_src = nullptr;
+  int rc = mem.alloc(&buf, sizeof(uint64_t) * (3 * words + 2 * nact + 1), "gbo_bfs_multisource");
+  if (rc || (rc = mem.alloc(&d_src, sizeof(int32_t) * nsrc, "gbo_bfs_multisource"))) return rc;
   uint64_t *F = buf, *Fn = buf + words, *V = buf + 2 * words;
   unsigned long long* act = (unsigned long long*)(buf + 3 * words);
   unsigned long long* act2 = act + nact;
   unsigned long long* cnt = (unsigned long long*)(buf + 3 * words + 2 * nact);
-  int32_t* d_src = nullptr;
-  GBO_HIP_CHECK(hipMallocAsync((void**)&d_src, sizeof(int32_t) * nsrc, st));
   GBO_HIP_CHECK(hipMemcpyAsync(d_src, h_src, sizeof(int32_t) * nsrc, hipMemcpyHostToDevice, st));
   GBO_HIP_CHECK(hipMemsetAsync(buf, 0, sizeof(uint64_t) * (3 * words + 2 * nact + 1), st));
   GBO_HIP_CHECK(hipMemsetAsync(d_level, 0xFF, sizeof(int32_t) * (size_t)nsrc * n, st));
   k_ms_init<KW><<<(nsrc + 255) / 256, 256, 0, st>>>(d_src, nsrc, F, V, d_level, n, A->bs, act);
   GBO_LAUNCH_CHECK();
-  PlanView pv;
-  int rc = acquire_plan(A, &pv, st);
-  if (rc) return rc;
+  PlanGuard g(st);
+  if ((rc = acquire_plan(A, &g.pv, st))) return rc;
+  const PlanView& pv = g.pv;
   MsArgs a{};
   a.rowptr = A->rowptr;
   a.colind = A->colind;
@@ -250,10 +251,6 @@ static int msbfs_t(const gbo_bsr* A, const int32_t* h_src, int32_t nsrc, int32_t
     std::swap(act, act2);
   }
   if (h_nlevels) *h_nlevels = lvl + 1;
-  release_plan(&pv, st);
-  (void)hipFreeAsync(buf, st);
-  (void)hipFreeAsync(d_src, st);
-  GBO_HIP_CHECK(hipStreamSynchronize(st));
   return GBO_OK;
 }
 

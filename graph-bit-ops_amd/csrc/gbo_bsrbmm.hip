@@ -283,32 +283,32 @@ using namespace gbo;
 // Block-CSC of a block structure on the device (csr2csc on blocks): a stable
 // radix sort of the tile indices by block column.  The tiles are stored in
 // block-row order, so every column's list comes out with ascending rows.
-static int block_transpose(const gbo_bsr* B, int32_t** d_ptr, int32_t** d_row, int32_t** d_tile, hipStream_t st) {
+// The three outputs are owned by `mem` (the caller's allocations).
+static int block_transpose(const gbo_bsr* B, AsyncAllocs& mem, int32_t** d_ptr, int32_t** d_row, int32_t** d_tile,
+                           hipStream_t st) {
   const int32_t nbr = B->nblockrows, nbc = B->nblockcols;
   const int64_t T = B->nblocks;
-  *d_ptr = *d_row = *d_tile = nullptr;
+  const size_t tb = sizeof(int32_t) * (size_t)std::max<int64_t>(T, 1);
+  int rc;
+  if ((rc = mem.alloc(d_ptr, sizeof(int32_t) * ((size_t)nbc + 1), "block_transpose")) ||
+      (rc = mem.alloc(d_row, tb, "block_transpose")) || (rc = mem.alloc(d_tile, tb, "block_transpose")))
+    return rc;
+  AsyncAllocs tmpmem(st);
   int32_t *keys = nullptr, *idx = nullptr;
   void* tmp = nullptr;
   size_t tmp_bytes = 0;
   unsigned bits = 1;
   while (bits < 31 && (1ll << bits) <= nbc) ++bits;
-  GBO_HIP_CHECK(hipMallocAsync((void**)d_ptr, sizeof(int32_t) * ((size_t)nbc + 1), st));
-  GBO_HIP_CHECK(hipMallocAsync((void**)d_row, sizeof(int32_t) * std::max<int64_t>(T, 1), st));
-  GBO_HIP_CHECK(hipMallocAsync((void**)d_tile, sizeof(int32_t) * std::max<int64_t>(T, 1), st));
-  GBO_HIP_CHECK(hipMallocAsync((void**)&keys, sizeof(int32_t) * std::max<int64_t>(T, 1), st));
-  GBO_HIP_CHECK(hipMallocAsync((void**)&idx, sizeof(int32_t) * std::max<int64_t>(T, 1), st));
+  if ((rc = tmpmem.alloc(&keys, tb, "block_transpose")) || (rc = tmpmem.alloc(&idx, tb, "block_transpose"))) return rc;
   k_iota_i32<<<(unsigned)((T + 255) / 256), 256, 0, st>>>(idx, T);
   GBO_LAUNCH_CHECK();
   GBO_HIP_CHECK(rocprim::radix_sort_pairs(nullptr, tmp_bytes, B->colind, keys, idx, *d_tile, (size_t)T, 0, bits, st));
-  GBO_HIP_CHECK(hipMallocAsync(&tmp, std::max<size_t>(tmp_bytes, 1), st));
+  if ((rc = tmpmem.alloc(&tmp, std::max<size_t>(tmp_bytes, 1), "block_transpose"))) return rc;
   GBO_HIP_CHECK(rocprim::radix_sort_pairs(tmp, tmp_bytes, B->colind, keys, idx, *d_tile, (size_t)T, 0, bits, st));
   k_col_bounds<<<(unsigned)(((int64_t)nbc + 1 + 255) / 256), 256, 0, st>>>(keys, T, nbc, *d_ptr);
   GBO_LAUNCH_CHECK();
   k_tile_block_row<<<(unsigned)((T + 255) / 256), 256, 0, st>>>(B->rowptr, nbr, *d_tile, T, *d_row);
   GBO_LAUNCH_CHECK();
-  (void)hipFreeAsync(tmp, st);
-  (void)hipFreeAsync(keys, st);
-  (void)hipFreeAsync(idx, st);
   return GBO_OK;
 }
 
@@ -328,13 +328,13 @@ int gbo_bsrbmm_tc(const gbo_bsr* L, const gbo_bsr* LT, uint64_t* d_count, gbo_st
   hipStream_t st = stream;
   GBO_HIP_CHECK(hipMemsetAsync(d_count, 0, sizeof(uint64_t), st));
   if (L->nblocks == 0 || LT->nblocks == 0) return GBO_OK;
+  AsyncAllocs mem(st);
   int32_t *bt_ptr = nullptr, *bt_row = nullptr, *bt_tile = nullptr;
-  rc = block_transpose(LT, &bt_ptr, &bt_row, &bt_tile, st);
-  if (rc) return rc;
+  if ((rc = block_transpose(LT, mem, &bt_ptr, &bt_row, &bt_tile, st))) return rc;
   // B's tiles in row form (same positions), for one popcount per set bit of A
   void* rows = nullptr;
   const size_t tbytes = (size_t)LT->nblocks * LT->bs * LT->bs / 8;
-  GBO_HIP_CHECK(hipMallocAsync(&rows, std::max<size_t>(tbytes, 8), st));
+  if ((rc = mem.alloc(&rows, std::max<size_t>(tbytes, 8), "gbo_bsrbmm_tc"))) return rc;
   {
     const int64_t waves = (LT->nblocks * LT->bs + kWave - 1) / kWave;
     const unsigned tb = (unsigned)((waves + 3) / 4);
@@ -346,41 +346,37 @@ int gbo_bsrbmm_tc(const gbo_bsr* L, const gbo_bsr* LT, uint64_t* d_count, gbo_st
     }
     GBO_LAUNCH_CHECK();
   }
-  // TC work per mask tile is far larger than a bsrbmv tile: small chunks.
+  // TC work per mask tile is far larger than a bsrbmv tile: small chunks
+  // (a plan without the bsrbmv scratch).
   gbo_bsr tmp = *L;
   tmp.plan = nullptr;
   tmp.plan_chunk = 0;
-  rc = gbo_bsr_plan(&tmp, 16, st);
-  if (rc == GBO_OK) {
-    TcArgs a{};
-    a.a_rowptr = L->rowptr;
-    a.a_colind = L->colind;
-    a.a_tiles = L->tiles;
-    a.nbr = L->nblockrows;
-    a.T = L->nblocks;
-    a.plan = tmp.plan;
-    a.chunk = tmp.plan_chunk;
-    a.nchunks = tmp.plan_nchunks;
-    a.bt_ptr = bt_ptr;
-    a.bt_row = bt_row;
-    a.bt_tile = bt_tile;
-    a.b_tiles = rows;
-    a.count = (unsigned long long*)d_count;
-    const unsigned blocks = (unsigned)(((int64_t)a.nchunks + 3) / 4);
-    switch (L->bs) {
-      case 8: k_tc<8><<<blocks, 256, 0, st>>>(a); break;
-      case 16: k_tc<16><<<blocks, 256, 0, st>>>(a); break;
-      case 32: k_tc<32><<<blocks, 256, 0, st>>>(a); break;
-      case 64: k_tc<64><<<blocks, 256, 0, st>>>(a); break;
-    }
-    if (hipGetLastError() != hipSuccess) rc = set_error(GBO_EDEVICE, "k_tc launch failed");
+  if ((rc = plan_create(&tmp, 16, 0, false, st))) return rc;
+  TcArgs a{};
+  a.a_rowptr = L->rowptr;
+  a.a_colind = L->colind;
+  a.a_tiles = L->tiles;
+  a.nbr = L->nblockrows;
+  a.T = L->nblocks;
+  a.plan = tmp.plan;
+  a.chunk = tmp.plan_chunk;
+  a.nchunks = tmp.plan_nchunks;
+  a.bt_ptr = bt_ptr;
+  a.bt_row = bt_row;
+  a.bt_tile = bt_tile;
+  a.b_tiles = rows;
+  a.count = (unsigned long long*)d_count;
+  const unsigned blocks = (unsigned)(((int64_t)a.nchunks + 3) / 4);
+  switch (L->bs) {
+    case 8: k_tc<8><<<blocks, 256, 0, st>>>(a); break;
+    case 16: k_tc<16><<<blocks, 256, 0, st>>>(a); break;
+    case 32: k_tc<32><<<blocks, 256, 0, st>>>(a); break;
+    case 64: k_tc<64><<<blocks, 256, 0, st>>>(a); break;
   }
+  if (hipGetLastError() != hipSuccess) rc = set_error(GBO_EDEVICE, "k_tc launch failed");
+  // the plan is a plain allocation: wait for the kernel before freeing it
   (void)hipStreamSynchronize(st);
   gbo_bsr_plan_free(&tmp);
-  (void)hipFreeAsync(bt_ptr, st);
-  (void)hipFreeAsync(bt_row, st);
-  (void)hipFreeAsync(bt_tile, st);
-  (void)hipFreeAsync(rows, st);
   return rc;
   GBO_TRY_END
 }
@@ -396,8 +392,9 @@ int gbo_bsrbmm_sum(const gbo_bsr* A, const gbo_bsr* B, uint64_t* d_sum, gbo_stre
   hipStream_t st = stream;
   const int bs = A->bs;
   const int64_t n = (int64_t)A->nblockcols * bs;
+  AsyncAllocs mem(st);
   unsigned long long* S = nullptr;
-  GBO_HIP_CHECK(hipMallocAsync((void**)&S, sizeof(unsigned long long) * 2 * std::max<int64_t>(n, 1), st));
+  if ((rc = mem.alloc(&S, sizeof(unsigned long long) * 2 * std::max<int64_t>(n, 1), "gbo_bsrbmm_sum"))) return rc;
   GBO_HIP_CHECK(hipMemsetAsync(S, 0, sizeof(unsigned long long) * 2 * std::max<int64_t>(n, 1), st));
   GBO_HIP_CHECK(hipMemsetAsync(d_sum, 0, sizeof(uint64_t), st));
   auto launch = [&](const gbo_bsr* M, bool by_row, unsigned long long* out) {
@@ -415,7 +412,6 @@ int gbo_bsrbmm_sum(const gbo_bsr* A, const gbo_bsr* B, uint64_t* d_sum, gbo_stre
   GBO_LAUNCH_CHECK();
   k_dot_u64<<<256, 256, 0, st>>>(S, S + n, n, (unsigned long long*)d_sum);
   GBO_LAUNCH_CHECK();
-  GBO_HIP_CHECK(hipFreeAsync(S, st));
   return GBO_OK;
 }
 

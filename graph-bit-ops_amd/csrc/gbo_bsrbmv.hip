@@ -1464,11 +1464,6 @@ static SpmvArgs base_args(const gbo_bsr* A, const PlanView& pv) {
   return a;
 }
 
-// Runs one operation with a plan acquired (and released) around it.
-__global__ __launch_bounds__(256) void k_zero_u64(unsigned long long* p, int n) {
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0ull;
-}
-
 // One block: *out = sum of the slots; the slots are cleared for the next use.
 __global__ __launch_bounds__(256) void k_count_fold(unsigned long long* slots, int64_t* out) {
   __shared__ unsigned long long sh[256];
@@ -1486,29 +1481,23 @@ __global__ __launch_bounds__(256) void k_count_fold(unsigned long long* slots, i
   if (threadIdx.x == 0) *out = (int64_t)sh[0];
 }
 
-int count_slots_create(unsigned long long** slots, hipStream_t st) {
-  GBO_HIP_CHECK(hipMallocAsync((void**)slots, sizeof(unsigned long long) * kCountSlots, st));
-  k_zero_u64<<<kCountSlots / 256, 256, 0, st>>>(*slots, kCountSlots);
-  GBO_LAUNCH_CHECK();
-  return GBO_OK;
-}
-
 int count_fold(unsigned long long* slots, int64_t* d_count, hipStream_t st) {
   k_count_fold<<<1, 256, 0, st>>>(slots, d_count);
   GBO_LAUNCH_CHECK();
   return GBO_OK;
 }
 
+// Runs one operation with a plan acquired (and released) around it; `fill`
+// sets the operands and may take scratch from the plan view.
 template <class Fill>
 static int run_op(const gbo_bsr* A, int op, hipStream_t st, Fill fill) {
-  PlanView pv;
-  int rc = acquire_plan(A, &pv, st);
+  PlanGuard g(st);
+  int rc = acquire_plan(A, &g.pv, st);
   if (rc) return rc;
-  SpmvArgs a = base_args(A, pv);
-  rc = fill(a);
+  SpmvArgs a = base_args(A, g.pv);
+  rc = fill(a, g.pv);
   if (rc == GBO_OK) rc = spmv_launch(A->bs, op, a, st);
-  int rc2 = release_plan(&pv, st);
-  return rc ? rc : rc2;
+  return rc;
 }
 
 }  // namespace gbo
@@ -1521,7 +1510,7 @@ int gbo_bsrbmv_count(const gbo_bsr* A, const void* d_xbits, int32_t* d_y, int ac
   int rc = validate_bsr(A, 0, "gbo_bsrbmv_count");
   if (rc) return rc;
   if (A->nrows > 0 && (!d_y || !d_xbits)) return set_error(GBO_EINVAL, "gbo_bsrbmv_count: null vector");
-  return run_op(A, OP_COUNT, stream, [&](SpmvArgs& a) -> int {
+  return run_op(A, OP_COUNT, stream, [&](SpmvArgs& a, const PlanView&) -> int {
     a.xbits = d_xbits;
     a.out = d_y;
     a.accumulate = accumulate ? 1 : 0;
@@ -1529,29 +1518,25 @@ int gbo_bsrbmv_count(const gbo_bsr* A, const void* d_xbits, int32_t* d_y, int ac
   });
 }
 
+// The frontier-size slots, and at bs 8 under variant 3 the per-row counts
+// of the dense step, come from the plan's scratch; the plan is held until
+// the fold of the slots is on the stream.
 int gbo_bsrbmv_bfs_step(const gbo_bsr* A, const void* d_frontier, const void* d_visited, void* d_next,
                         int64_t* d_count, gbo_stream_t stream) {
   int rc = validate_bsr(A, 0, "gbo_bsrbmv_bfs_step");
   if (rc) return rc;
   if (!d_frontier || !d_visited || !d_next || !d_count) return set_error(GBO_EINVAL, "gbo_bsrbmv_bfs_step: null argument");
-  unsigned long long* slots = nullptr;
-  rc = count_slots_create(&slots, stream);
+  PlanGuard g(stream);
+  rc = acquire_plan(A, &g.pv, stream);
   if (rc) return rc;
-  int32_t* counts = nullptr;
-  rc = run_op(A, OP_BFS, stream, [&](SpmvArgs& a) -> int {
-    a.xbits = d_frontier;
-    a.visited = d_visited;
-    a.out = d_next;
-    a.count_slots = slots;
-    if (A->bs == 8 && a.variant == 3) {
-      GBO_HIP_CHECK(hipMallocAsync((void**)&counts, sizeof(int32_t) * 8 * ((size_t)A->nblockrows + 1), stream));
-      a.bfs_counts = counts;
-    }
-    return GBO_OK;
-  });
-  if (rc == GBO_OK) rc = count_fold(slots, d_count, stream);
-  if (counts) (void)hipFreeAsync(counts, stream);
-  (void)hipFreeAsync(slots, stream);
+  SpmvArgs a = base_args(A, g.pv);
+  a.xbits = d_frontier;
+  a.visited = d_visited;
+  a.out = d_next;
+  a.count_slots = g.pv.slots;
+  if (A->bs == 8 && a.variant == 3) a.bfs_counts = g.pv.bfs_counts;
+  rc = spmv_launch(A->bs, OP_BFS, a, stream);
+  if (rc == GBO_OK) rc = count_fold(g.pv.slots, d_count, stream);
   return rc;
 }
 
@@ -1562,7 +1547,7 @@ int gbo_bsrbmv_minplus(const gbo_bsr* A, const uint32_t* d_w, const int64_t* d_w
   if (A->nrows != A->ncols) return set_error(GBO_EINVAL, "gbo_bsrbmv_minplus: matrix must be square");
   if (A->nrows > 0 && (!d_dist || !d_out)) return set_error(GBO_EINVAL, "gbo_bsrbmv_minplus: null vector");
   if ((d_w == nullptr) != (d_woff == nullptr)) return set_error(GBO_EINVAL, "gbo_bsrbmv_minplus: weights need offsets");
-  return run_op(A, OP_MINPLUS, stream, [&](SpmvArgs& a) -> int {
+  return run_op(A, OP_MINPLUS, stream, [&](SpmvArgs& a, const PlanView&) -> int {
     a.w = d_w;
     a.woff = d_woff;
     a.xval = d_dist;
@@ -1571,27 +1556,24 @@ int gbo_bsrbmv_minplus(const gbo_bsr* A, const uint32_t* d_w, const int64_t* d_w
   });
 }
 
+// The FP64 split-row partials come from the plan's scratch.
 int gbo_bsrbmv_sum_f32(const gbo_bsr* A, const float* d_x, float* d_y, gbo_stream_t stream) {
   int rc = validate_bsr(A, 0, "gbo_bsrbmv_sum_f32");
   if (rc) return rc;
   if (A->nrows > 0 && (!d_x || !d_y)) return set_error(GBO_EINVAL, "gbo_bsrbmv_sum_f32: null vector");
-  double* partial = nullptr;
-  rc = run_op(A, OP_SUM, stream, [&](SpmvArgs& a) -> int {
-    GBO_HIP_CHECK(hipMallocAsync((void**)&partial, sizeof(double) * 2 * (size_t)a.nchunks * A->bs, stream));
+  return run_op(A, OP_SUM, stream, [&](SpmvArgs& a, const PlanView& pv) -> int {
     a.xval = d_x;
     a.out = d_y;
-    a.partial = partial;
+    a.partial = pv.partial;
     return GBO_OK;
   });
-  if (partial) (void)hipFreeAsync(partial, stream);
-  return rc;
 }
 
 int gbo_bsrbmv_min_i32(const gbo_bsr* A, const int32_t* d_x, int32_t* d_y, gbo_stream_t stream) {
   int rc = validate_bsr(A, 0, "gbo_bsrbmv_min_i32");
   if (rc) return rc;
   if (A->nrows > 0 && (!d_x || !d_y)) return set_error(GBO_EINVAL, "gbo_bsrbmv_min_i32: null vector");
-  return run_op(A, OP_MIN, stream, [&](SpmvArgs& a) -> int {
+  return run_op(A, OP_MIN, stream, [&](SpmvArgs& a, const PlanView&) -> int {
     a.xval = d_x;
     a.out = d_y;
     return GBO_OK;

@@ -63,6 +63,8 @@ struct gbo_batch {
   hipGraphExec_t exec = nullptr;
   int32_t* graph_level = nullptr;
   int32_t* graph_label = nullptr;
+  int64_t graph_spin_ms = -1;  // barrier wait limit baked into the captured launch
+  int64_t spin_ms = 0;         // limit of the last launch (direct or replayed)
 };
 
 namespace gbo {
@@ -807,7 +809,7 @@ static int64_t batch_spin_ms() {
   return v < 0 ? 0 : (int64_t)v;
 }
 
-static int batch_launch(gbo_batch* b, int32_t* d_level, int32_t* d_label, hipStream_t st) {
+static int batch_launch(gbo_batch* b, int32_t* d_level, int32_t* d_label, int64_t spin_ms, hipStream_t st) {
   BatchArgs a{};
   a.rowptr = b->A.rowptr;
   a.colind = b->A.colind;
@@ -832,7 +834,7 @@ static int batch_launch(gbo_batch* b, int32_t* d_level, int32_t* d_label, hipStr
   int dev = 0, khz = 0;
   GBO_HIP_CHECK(hipGetDevice(&dev));
   GBO_HIP_CHECK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
-  a.spin_ticks = (int64_t)std::max(khz, 1) * batch_spin_ms();
+  a.spin_ticks = (int64_t)std::max(khz, 1) * spin_ms;
   a.join_ticks = (int64_t)std::max(khz, 1) * kBatchJoinUs / 1000;
   k_batch_reset<<<1, 1, 0, st>>>(b->d_sync);
   int rc = GBO_OK;
@@ -854,7 +856,7 @@ static int batch_check(gbo_batch* b) {
   GBO_HIP_CHECK(hipMemcpy(&abort, &b->d_sync->abort, sizeof(abort), hipMemcpyDeviceToHost));
   if (abort)
     return set_error(GBO_EDEVICE, "gbo_batch_bfs_cc: grid barrier wait exceeded %lld ms",
-                     (long long)batch_spin_ms());
+                     (long long)b->spin_ms);
   return GBO_OK;
 }
 
@@ -968,13 +970,17 @@ int gbo_batch_bfs_cc(gbo_batch* b, const int32_t* h_sources, int32_t* d_level, i
   }
   hipStream_t st = stream;
   GBO_HIP_CHECK(hipMemcpyAsync(b->d_src, h_sources, sizeof(int32_t) * b->ngraphs, hipMemcpyHostToDevice, st));
+  const int64_t spin_ms = batch_spin_ms();
   if (!use_graph) {
-    int rc = batch_launch(b, d_level, d_label, st);
+    int rc = batch_launch(b, d_level, d_label, spin_ms, st);
     if (rc) return rc;
+    b->spin_ms = spin_ms;
     GBO_HIP_CHECK(hipStreamSynchronize(st));
     return batch_check(b);
   }
-  if (b->exec && (b->graph_level != d_level || b->graph_label != d_label)) {
+  // the outputs and the wait limit are baked into the captured launch:
+  // capture again when either differs
+  if (b->exec && (b->graph_level != d_level || b->graph_label != d_label || b->graph_spin_ms != spin_ms)) {
     (void)hipGraphExecDestroy(b->exec);
     (void)hipGraphDestroy(b->graph);
     b->exec = nullptr;
@@ -986,7 +992,7 @@ int gbo_batch_bfs_cc(gbo_batch* b, const int32_t* h_sources, int32_t* d_level, i
     GBO_HIP_CHECK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
     GBO_HIP_CHECK(hipStreamSynchronize(st));
     GBO_HIP_CHECK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
-    int rc = batch_launch(b, d_level, d_label, cs);
+    int rc = batch_launch(b, d_level, d_label, spin_ms, cs);
     hipGraph_t gr = nullptr;
     hipError_t e = hipStreamEndCapture(cs, &gr);
     (void)hipStreamDestroy(cs);
@@ -996,7 +1002,9 @@ int gbo_batch_bfs_cc(gbo_batch* b, const int32_t* h_sources, int32_t* d_level, i
     GBO_HIP_CHECK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
     b->graph_level = d_level;
     b->graph_label = d_label;
+    b->graph_spin_ms = spin_ms;
   }
+  b->spin_ms = b->graph_spin_ms;
   GBO_HIP_CHECK(hipGraphLaunch(b->exec, st));
   GBO_HIP_CHECK(hipStreamSynchronize(st));
   return batch_check(b);

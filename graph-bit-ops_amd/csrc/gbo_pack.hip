@@ -252,6 +252,10 @@ __global__ __launch_bounds__(256) void k_plan(const int32_t* __restrict__ rowptr
   plan[c] = R;
 }
 
+__global__ __launch_bounds__(256) void k_fill_u64(unsigned long long* p, int n, unsigned long long v) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
+}
+
 int default_chunk(int bs) {
   switch (bs) {
     case 8: return 256;
@@ -291,8 +295,53 @@ static int dispatch_pack(int bs, const int32_t* rp, const int32_t* ci, int32_t n
   return set_error(GBO_EUNSUPPORTED, "block size %d not supported (8, 16, 32, 64)", bs);
 }
 
-// Plan for a matrix: uses A->plan when present, else builds a temporary one
-// (stream-ordered allocation, released by release_plan).
+static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+PlanLayout plan_layout(int bs, int64_t nchunks, int32_t nbr, bool scratch) {
+  PlanLayout L;
+  size_t off = align256(sizeof(int32_t) * (size_t)nchunks);
+  if (scratch) {
+    L.slots = off;
+    off += align256(sizeof(unsigned long long) * kCountSlots);
+    L.count = off;
+    off += 256;
+    L.partial = off;
+    off += align256(sizeof(double) * 2 * (size_t)nchunks * bs);
+    if (bs == 8) {
+      L.bfs_counts = off;
+      off += align256(sizeof(int32_t) * 8 * ((size_t)nbr + 1));
+    }
+  }
+  L.bytes = off;
+  return L;
+}
+
+// Points the view's scratch pointers into a plan allocation made with scratch.
+static void view_scratch(PlanView* pv, int bs, int32_t nbr) {
+  const PlanLayout L = plan_layout(bs, pv->nchunks, nbr, true);
+  char* base = (char*)const_cast<int32_t*>(pv->plan);
+  pv->slots = (unsigned long long*)(base + L.slots);
+  pv->count = (int64_t*)(base + L.count);
+  pv->partial = (double*)(base + L.partial);
+  pv->bfs_counts = bs == 8 ? (int32_t*)(base + L.bfs_counts) : nullptr;
+}
+
+// Plan entries plus (optionally) zeroed count slots in one allocation.
+static int plan_fill(const gbo_bsr* A, int32_t chunk, int64_t nch, bool scratch, int32_t* plan, hipStream_t st) {
+  k_plan<<<(unsigned)((nch + 255) / 256), 256, 0, st>>>(A->rowptr, A->nblockrows, chunk, (int32_t)nch, plan);
+  GBO_LAUNCH_CHECK();
+  if (scratch) {
+    const PlanLayout L = plan_layout(A->bs, nch, A->nblockrows, true);
+    unsigned long long* slots = (unsigned long long*)((char*)plan + L.slots);
+    k_fill_u64<<<kCountSlots / 256, 256, 0, st>>>(slots, kCountSlots, 0ull);
+    GBO_LAUNCH_CHECK();
+  }
+  return GBO_OK;
+}
+
+// Plan for a matrix: uses A->plan (made by gbo_bsr_plan, with its scratch)
+// when present, else builds a temporary one with scratch (stream-ordered
+// allocation, released by release_plan).
 int acquire_plan(const gbo_bsr* A, PlanView* pv, hipStream_t st) {
   if (A->plan && A->plan_chunk > 0 && A->plan_nchunks > 0) {
     pv->plan = A->plan;
@@ -300,21 +349,37 @@ int acquire_plan(const gbo_bsr* A, PlanView* pv, hipStream_t st) {
     pv->nchunks = A->plan_nchunks;
     pv->variant = A->plan_variant;
     pv->owned = false;
+    view_scratch(pv, A->bs, A->nblockrows);
     return GBO_OK;
   }
   const int32_t chunk = default_chunk(A->bs);
   const int64_t nch = std::max<int64_t>(1, (A->nblocks + chunk - 1) / chunk);
   if (nch >= INT_MAX) return set_error(GBO_ERANGE, "too many chunks");
+  const PlanLayout L = plan_layout(A->bs, nch, A->nblockrows, true);
   int32_t* plan = nullptr;
-  GBO_HIP_CHECK(hipMallocAsync((void**)&plan, sizeof(int32_t) * nch, st));
-  k_plan<<<(unsigned)((nch + 255) / 256), 256, 0, st>>>(A->rowptr, A->nblockrows, chunk, (int32_t)nch, plan);
-  GBO_LAUNCH_CHECK();
+  GBO_HIP_CHECK(hipMallocAsync((void**)&plan, L.bytes, st));
   pv->plan = plan;
   pv->chunk = chunk;
   pv->nchunks = (int32_t)nch;
   pv->variant = A->plan_variant;
   pv->owned = true;
-  return GBO_OK;
+  view_scratch(pv, A->bs, A->nblockrows);
+  return plan_fill(A, chunk, nch, true, plan, st);
+}
+
+int plan_create(gbo_bsr* A, int32_t chunk_tiles, int32_t variant, bool scratch, hipStream_t st) {
+  const int32_t chunk = chunk_tiles > 0 ? chunk_tiles : default_chunk(A->bs);
+  const int64_t nch = std::max<int64_t>(1, (A->nblocks + chunk - 1) / chunk);
+  if (nch >= INT_MAX) return set_error(GBO_ERANGE, "gbo_bsr_plan: too many chunks");
+  gbo_bsr_plan_free(A);
+  const PlanLayout L = plan_layout(A->bs, nch, A->nblockrows, scratch);
+  GBO_HIP_CHECK(hipMalloc((void**)&A->plan, L.bytes));
+  A->plan_chunk = chunk;
+  A->plan_nchunks = (int32_t)nch;
+  A->plan_variant = variant;
+  const int rc = plan_fill(A, chunk, nch, scratch, A->plan, st);
+  if (rc) gbo_bsr_plan_free(A);
+  return rc;
 }
 
 int release_plan(PlanView* pv, hipStream_t st) {
@@ -552,16 +617,7 @@ int gbo_bsr_plan_ex(gbo_bsr* A, int32_t chunk_tiles, int32_t variant, gbo_stream
   if (rc) return rc;
   if (chunk_tiles < 0) return set_error(GBO_EINVAL, "gbo_bsr_plan: negative chunk");
   if (variant < 0 || variant > 3) return set_error(GBO_EINVAL, "gbo_bsr_plan: variant must be 0, 1, 2 or 3");
-  const int32_t chunk = chunk_tiles > 0 ? chunk_tiles : default_chunk(A->bs);
-  const int64_t nch = std::max<int64_t>(1, (A->nblocks + chunk - 1) / chunk);
-  gbo_bsr_plan_free(A);
-  GBO_HIP_CHECK(hipMalloc((void**)&A->plan, sizeof(int32_t) * nch));
-  k_plan<<<(unsigned)((nch + 255) / 256), 256, 0, stream>>>(A->rowptr, A->nblockrows, chunk, (int32_t)nch, A->plan);
-  GBO_LAUNCH_CHECK();
-  A->plan_chunk = chunk;
-  A->plan_nchunks = (int32_t)nch;
-  A->plan_variant = variant;
-  return GBO_OK;
+  return plan_create(A, chunk_tiles, variant, true, stream);
 }
 
 int gbo_csr2bsr_values_u32(const gbo_bsr* A, const int32_t* d_rp, const int32_t* d_ci, const uint32_t* d_val,

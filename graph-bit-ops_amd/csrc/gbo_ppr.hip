@@ -360,12 +360,22 @@ __global__ void k_scale_copy(const float* __restrict__ src, float* __restrict__ 
 }
 
 // Plans with a chunk sized for the SpMM (one tile costs up to bs*K loads).
+// The SpMM's work plan and scratch, released on every return path.
 struct PprPlan {
   int32_t* plan = nullptr;
   int32_t chunk = 0, nchunks = 0;
   float* partial = nullptr;
   float* xs = nullptr;  // bs 8: X with the per-source scale applied, [n][K]
   int64_t xs_len = 0;
+  hipStream_t st = nullptr;
+  explicit PprPlan(hipStream_t s) : st(s) {}
+  ~PprPlan() {
+    if (plan) (void)hipFreeAsync(plan, st);
+    if (partial) (void)hipFreeAsync(partial, st);
+    if (xs) (void)hipFreeAsync(xs, st);
+  }
+  PprPlan(const PprPlan&) = delete;
+  PprPlan& operator=(const PprPlan&) = delete;
 };
 
 __global__ void k_plan_ppr(const int32_t* __restrict__ rowptr, int32_t nbr, int32_t chunk, int32_t nchunks,
@@ -394,16 +404,6 @@ static int make_ppr_plan(const gbo_bsr* M, PprPlan* P, hipStream_t st) {
   k_plan_ppr<<<(P->nchunks + 255) / 256, 256, 0, st>>>(M->rowptr, M->nblockrows, P->chunk, P->nchunks, P->plan);
   GBO_LAUNCH_CHECK();
   return GBO_OK;
-}
-
-static void free_ppr_plan(PprPlan* P, hipStream_t st) {
-  if (P->plan) (void)hipFreeAsync(P->plan, st);
-  if (P->partial) (void)hipFreeAsync(P->partial, st);
-  if (P->xs) (void)hipFreeAsync(P->xs, st);
-  P->plan = nullptr;
-  P->partial = nullptr;
-  P->xs = nullptr;
-  P->xs_len = 0;
 }
 
 // Y = a * rowscale * (M (colscale .* X)) + b * Z over all K columns.
@@ -511,45 +511,41 @@ int gbo_ppr_forward(const gbo_bsr* AT, const int32_t* d_outdeg, int32_t K, float
   hipStream_t st = stream;
   const int32_t n = AT->nrows;
   const int64_t nk = (int64_t)n * K;
+  AsyncAllocs mem(st);
   float *inv = nullptr, *q = nullptr;
   double* dpart = nullptr;
-  GBO_HIP_CHECK(hipMallocAsync((void**)&inv, sizeof(float) * std::max(n, 1), st));
-  GBO_HIP_CHECK(hipMallocAsync((void**)&q, sizeof(float) * std::max<int64_t>(nk, 1), st));
-  GBO_HIP_CHECK(hipMallocAsync((void**)&dpart, sizeof(double) * (kColredBlocks * 64 + 64), st));
+  if ((rc = mem.alloc(&inv, sizeof(float) * std::max(n, 1), "gbo_ppr_forward")) ||
+      (rc = mem.alloc(&q, sizeof(float) * std::max<int64_t>(nk, 1), "gbo_ppr_forward")) ||
+      (rc = mem.alloc(&dpart, sizeof(double) * (kColredBlocks * 64 + 64), "gbo_ppr_forward")))
+    return rc;
   k_invdeg<<<(n + 255) / 256, 256, 0, st>>>(d_outdeg, inv, n);
   GBO_LAUNCH_CHECK();
-  PprPlan P;
-  rc = make_ppr_plan(AT, &P, st);
+  PprPlan P(st);
+  if ((rc = make_ppr_plan(AT, &P, st))) return rc;
   // p_0 = s
-  if (rc == GBO_OK) GBO_HIP_CHECK(hipMemcpyAsync(d_p, d_s, sizeof(float) * nk, hipMemcpyDeviceToDevice, st));
+  GBO_HIP_CHECK(hipMemcpyAsync(d_p, d_s, sizeof(float) * nk, hipMemcpyDeviceToDevice, st));
   std::vector<double> delta(K), snorm(K);
   // the change is measured relative to ||s_k||_1 (1 for a distribution)
-  if (rc == GBO_OK) rc = colreduce(d_s, nullptr, nullptr, n, K, 2, dpart, dpart + kColredBlocks * 64, snorm.data(), st);
+  if ((rc = colreduce(d_s, nullptr, nullptr, n, K, 2, dpart, dpart + kColredBlocks * 64, snorm.data(), st))) return rc;
   int32_t it = 0;
   double dmax = 0.0;
   float* cur = d_p;
   float* nxt = q;
-  while (rc == GBO_OK && it < max_iter) {
-    rc = spmm(AT, P, K, cur, inv, nullptr, alpha, 1.0f - alpha, d_s, nxt, st);
-    if (rc) break;
-    rc = colreduce(nxt, cur, nullptr, n, K, 0, dpart, dpart + kColredBlocks * 64, delta.data(), st);
-    if (rc) break;
+  while (it < max_iter) {
+    if ((rc = spmm(AT, P, K, cur, inv, nullptr, alpha, 1.0f - alpha, d_s, nxt, st))) return rc;
+    if ((rc = colreduce(nxt, cur, nullptr, n, K, 0, dpart, dpart + kColredBlocks * 64, delta.data(), st))) return rc;
     ++it;
     std::swap(cur, nxt);
     dmax = max_relative(delta, snorm);
     if (dmax < tol) break;
   }
-  if (rc == GBO_OK && cur != d_p) GBO_HIP_CHECK(hipMemcpyAsync(d_p, cur, sizeof(float) * nk, hipMemcpyDeviceToDevice, st));
-  free_ppr_plan(&P, st);
-  (void)hipFreeAsync(inv, st);
-  (void)hipFreeAsync(q, st);
-  (void)hipFreeAsync(dpart, st);
+  if (cur != d_p) GBO_HIP_CHECK(hipMemcpyAsync(d_p, cur, sizeof(float) * nk, hipMemcpyDeviceToDevice, st));
   GBO_HIP_CHECK(hipStreamSynchronize(st));
   if (h_iters) *h_iters = it;
   if (h_delta) *h_delta = dmax;
-  if (rc == GBO_OK && !(dmax < tol))
+  if (!(dmax < tol))
     return set_error(GBO_ENOCONV, "gbo_ppr_forward: not converged after %d iterations (%.3e)", it, dmax);
-  return rc;
+  return GBO_OK;
 }
 
 int gbo_ppr_adjoint(const gbo_bsr* A, const gbo_bsr* AT, const int32_t* d_outdeg, int32_t K, float alpha,
@@ -567,59 +563,51 @@ int gbo_ppr_adjoint(const gbo_bsr* A, const gbo_bsr* AT, const int32_t* d_outdeg
   hipStream_t st = stream;
   const int32_t n = A->nrows;
   const int64_t nk = (int64_t)n * K;
-  float *inv = nullptr, *lam = nullptr, *q = nullptr;
+  AsyncAllocs mem(st);
+  float *inv = nullptr, *lam = nullptr;
   double* dpart = nullptr;
-  GBO_HIP_CHECK(hipMallocAsync((void**)&inv, sizeof(float) * std::max(n, 1), st));
-  GBO_HIP_CHECK(hipMallocAsync((void**)&lam, sizeof(float) * std::max<int64_t>(nk, 1) * 2, st));
-  q = lam + nk;
-  GBO_HIP_CHECK(hipMallocAsync((void**)&dpart, sizeof(double) * (kColredBlocks * 64 + 64), st));
+  if ((rc = mem.alloc(&inv, sizeof(float) * std::max(n, 1), "gbo_ppr_adjoint")) ||
+      (rc = mem.alloc(&lam, sizeof(float) * std::max<int64_t>(nk, 1) * 2, "gbo_ppr_adjoint")) ||
+      (rc = mem.alloc(&dpart, sizeof(double) * (kColredBlocks * 64 + 64), "gbo_ppr_adjoint")))
+    return rc;
+  float* q = lam + nk;
   k_invdeg<<<(n + 255) / 256, 256, 0, st>>>(d_outdeg, inv, n);
   GBO_LAUNCH_CHECK();
-  PprPlan PA, PT;
-  rc = make_ppr_plan(A, &PA, st);
-  if (rc == GBO_OK) rc = make_ppr_plan(AT, &PT, st);
-  if (rc == GBO_OK) GBO_HIP_CHECK(hipMemcpyAsync(lam, d_w, sizeof(float) * nk, hipMemcpyDeviceToDevice, st));
+  PprPlan PA(st), PT(st);
+  if ((rc = make_ppr_plan(A, &PA, st)) || (rc = make_ppr_plan(AT, &PT, st))) return rc;
+  GBO_HIP_CHECK(hipMemcpyAsync(lam, d_w, sizeof(float) * nk, hipMemcpyDeviceToDevice, st));
   std::vector<double> delta(K), ga(K), wnorm(K);
   // the change is measured relative to ||w_k||_1, the scale of lambda_k
-  if (rc == GBO_OK) rc = colreduce(d_w, nullptr, nullptr, n, K, 2, dpart, dpart + kColredBlocks * 64, wnorm.data(), st);
+  if ((rc = colreduce(d_w, nullptr, nullptr, n, K, 2, dpart, dpart + kColredBlocks * 64, wnorm.data(), st))) return rc;
   int32_t it = 0;
   float* cur = lam;
   float* nxt = q;
   double dmax = 0.0;
-  while (rc == GBO_OK && it < max_iter) {
+  while (it < max_iter) {
     // lambda' = alpha * D^-1 A lambda + w
-    rc = spmm(A, PA, K, cur, nullptr, inv, alpha, 1.0f, d_w, nxt, st);
-    if (rc) break;
-    rc = colreduce(nxt, cur, nullptr, n, K, 0, dpart, dpart + kColredBlocks * 64, delta.data(), st);
-    if (rc) break;
+    if ((rc = spmm(A, PA, K, cur, nullptr, inv, alpha, 1.0f, d_w, nxt, st))) return rc;
+    if ((rc = colreduce(nxt, cur, nullptr, n, K, 0, dpart, dpart + kColredBlocks * 64, delta.data(), st))) return rc;
     ++it;
     std::swap(cur, nxt);
     dmax = max_relative(delta, wnorm);
     if (dmax < tol) break;
   }
-  if (rc == GBO_OK) {
-    // grad_s = (1 - alpha) * lambda
-    k_scale_copy<<<(unsigned)((nk + 255) / 256), 256, 0, st>>>(cur, d_grad_s, nk, 1.0f - alpha);
-    GBO_LAUNCH_CHECK();
-    // dL/dalpha = sum_k lambda_k . (P^T p_k - s_k)
-    rc = spmm(AT, PT, K, d_p, inv, nullptr, 1.0f, 0.0f, nullptr, nxt, st);
-    if (rc == GBO_OK) rc = colreduce(nxt, d_s, cur, n, K, 1, dpart, dpart + kColredBlocks * 64, ga.data(), st);
-    if (rc == GBO_OK && h_grad_alpha) {
-      double s = 0.0;
-      for (double v : ga) s += v;
-      *h_grad_alpha = s;
-    }
+  // grad_s = (1 - alpha) * lambda
+  k_scale_copy<<<(unsigned)((nk + 255) / 256), 256, 0, st>>>(cur, d_grad_s, nk, 1.0f - alpha);
+  GBO_LAUNCH_CHECK();
+  // dL/dalpha = sum_k lambda_k . (P^T p_k - s_k)
+  if ((rc = spmm(AT, PT, K, d_p, inv, nullptr, 1.0f, 0.0f, nullptr, nxt, st))) return rc;
+  if ((rc = colreduce(nxt, d_s, cur, n, K, 1, dpart, dpart + kColredBlocks * 64, ga.data(), st))) return rc;
+  if (h_grad_alpha) {
+    double s = 0.0;
+    for (double v : ga) s += v;
+    *h_grad_alpha = s;
   }
-  free_ppr_plan(&PA, st);
-  free_ppr_plan(&PT, st);
-  (void)hipFreeAsync(inv, st);
-  (void)hipFreeAsync(lam, st);
-  (void)hipFreeAsync(dpart, st);
   GBO_HIP_CHECK(hipStreamSynchronize(st));
   if (h_iters) *h_iters = it;
-  if (rc == GBO_OK && !(dmax < tol))
+  if (!(dmax < tol))
     return set_error(GBO_ENOCONV, "gbo_ppr_adjoint: not converged after %d iterations (%.3e)", it, dmax);
-  return rc;
+  return GBO_OK;
 }
 
 }  // extern "C"

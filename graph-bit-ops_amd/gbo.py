@@ -181,6 +181,21 @@ def _ptr(a):
     raise TypeError(type(a))
 
 
+def hip_runtimes() -> list[str]:
+    """Distinct HIP runtime libraries (libamdhip64) mapped into this process.
+    Device pointers can pass between libgbo.so and another HIP user (torch)
+    only when there is one: torch wheels bundle their own copy, and the
+    dynamic loader reuses it for libgbo.so (same soname) only when torch was
+    loaded first."""
+    found = set()
+    with open("/proc/self/maps") as f:
+        for line in f:
+            path = line.split()[-1] if "/" in line else ""
+            if os.path.basename(path).startswith("libamdhip64.so"):
+                found.add(os.path.realpath(path))
+    return sorted(found)
+
+
 def device_count() -> int:
     n = c_int(0)
     _check(lib().gbo_device_count(byref(n)))
@@ -593,7 +608,7 @@ def bfs_multisource(AT: BsrMatrix, sources, out: DeviceBuffer | None = None):
     device and only nlevels is returned."""
     src = np.ascontiguousarray(sources, np.int32)
     n = AT.nrows
-    d = out if out is not None else DeviceBuffer(4 * max(len(src) * n, 1))
+    d = _out_buffer(out, 4 * len(src) * n)
     nl = c_int32()
     _check(lib().gbo_bfs_multisource(AT.ref, _ptr(src), len(src), d.ptr, byref(nl), None))
     if out is not None:
@@ -616,6 +631,32 @@ def ppr_forward(AT: BsrMatrix, outdeg, S, alpha: float = 0.85, tol: float = 1e-7
     if rc != GBO_OK and (strict or rc != -7):
         _check(rc)
     return dp.to_numpy(np.float32, n * K).reshape(n, K), it.value, delta.value
+
+
+def ppr_forward_ptr(AT: BsrMatrix, d_outdeg: int, K: int, alpha: float, d_s: int, d_p: int, tol: float = 1e-7,
+                    max_iter: int = 1000, strict: bool = True, stream: int | None = None):
+    """gbo_ppr_forward on caller-owned device memory (raw device addresses of
+    [n][K] float32 S and P and of int32 out-degrees), stream-ordered on
+    `stream`: nothing is copied to or from the host.  Returns (iterations,
+    last relative change)."""
+    it, delta = c_int32(), c_double()
+    rc = lib().gbo_ppr_forward(AT.ref, d_outdeg, K, alpha, d_s, d_p, tol, max_iter, byref(it), byref(delta), stream)
+    if rc != GBO_OK and (strict or rc != -7):
+        _check(rc)
+    return it.value, delta.value
+
+
+def ppr_adjoint_ptr(A: BsrMatrix, AT: BsrMatrix, d_outdeg: int, K: int, alpha: float, d_s: int, d_p: int, d_w: int,
+                    d_grad_s: int, tol: float = 1e-7, max_iter: int = 1000, strict: bool = True,
+                    stream: int | None = None):
+    """gbo_ppr_adjoint on caller-owned device memory (as ppr_forward_ptr);
+    grad_S goes to d_grad_s.  Returns (grad_alpha, iterations)."""
+    ga, it = c_double(), c_int32()
+    rc = lib().gbo_ppr_adjoint(A.ref, AT.ref, d_outdeg, K, alpha, d_s, d_p, d_w, d_grad_s, byref(ga), tol, max_iter,
+                               byref(it), stream)
+    if rc != GBO_OK and (strict or rc != -7):
+        _check(rc)
+    return ga.value, it.value
 
 
 def ppr_adjoint(A: BsrMatrix, AT: BsrMatrix, outdeg, S, P, W, alpha: float = 0.85, tol: float = 1e-7,

@@ -509,3 +509,102 @@ uint64_t or_tc(int32_t n, const int32_t* rp, const int32_t* ci) {
   }
   return total;
 }
+
+/* ----------------------------------------------- personalized PageRank -- */
+/* The map of algorithm/pr.cu:449-516 (p' = alpha * sum p[j]/outdeg[j] +
+ * teleport) with a per-column personalization vector, and its adjoint; see
+ * oracle.h.  y is the product term for row i and column k. */
+static inline double ppr_y(int mode, const int32_t* rp, const int32_t* ci, const int32_t* outdeg, int32_t i,
+                           int32_t K, int32_t k, const double* Xd, const float* Xf) {
+  double y = 0.0;
+  for (int32_t q = rp[i]; q < rp[i + 1]; ++q) {
+    const int64_t j = ci[q];
+    const double x = Xd ? Xd[j * K + k] : (double)Xf[j * K + k];
+    if (mode == 0) {
+      if (outdeg[j] > 0) y += x / (double)outdeg[j];
+    } else {
+      y += x;
+    }
+  }
+  if (mode == 1) y = outdeg[i] > 0 ? y / (double)outdeg[i] : 0.0;
+  return y;
+}
+
+int32_t or_ppr_solve(int32_t n, const int32_t* rp, const int32_t* ci, const int32_t* outdeg, double alpha, int32_t K,
+                     int mode, const double* R, double* X, double tol, int32_t max_iter) {
+  const double c = mode == 0 ? 1.0 - alpha : 1.0;
+  double* Y = (double*)malloc(sizeof(double) * (size_t)n * K + 8);
+  double* rn = (double*)calloc((size_t)K, sizeof(double));
+  double* d = (double*)calloc((size_t)K, sizeof(double));
+  for (int64_t i = 0; i < n; ++i)
+    for (int32_t k = 0; k < K; ++k) rn[k] += fabs(R[i * K + k]);
+  int32_t it = 0, done = 0;
+  while (!done && it < max_iter) {
+    for (int32_t k = 0; k < K; ++k) d[k] = 0.0;
+#pragma omp parallel
+    {
+      double* dl = (double*)calloc((size_t)K, sizeof(double));
+#pragma omp for schedule(dynamic, 1024)
+      for (int32_t i = 0; i < n; ++i)
+        for (int32_t k = 0; k < K; ++k) {
+          const double t = alpha * ppr_y(mode, rp, ci, outdeg, i, K, k, X, NULL) + c * R[(int64_t)i * K + k];
+          dl[k] += fabs(t - X[(int64_t)i * K + k]);
+          Y[(int64_t)i * K + k] = t;
+        }
+#pragma omp critical
+      for (int32_t k = 0; k < K; ++k) d[k] += dl[k];
+      free(dl);
+    }
+    memcpy(X, Y, sizeof(double) * (size_t)n * K);
+    ++it;
+    double m = 0.0;
+    for (int32_t k = 0; k < K; ++k) {
+      const double rel = rn[k] > 0.0 ? d[k] / rn[k] : d[k];
+      if (rel > m) m = rel;
+    }
+    done = m < tol;
+  }
+  free(Y);
+  free(rn);
+  free(d);
+  return done ? it : -1;
+}
+
+void or_ppr_check(int32_t n, const int32_t* rp, const int32_t* ci, const int32_t* outdeg, double alpha, int32_t K,
+                  int mode, const float* X, const float* R, const float* L, double* res_l1, double* res_inf,
+                  double* dot, double* y_l1) {
+  const double c = mode == 0 ? 1.0 - alpha : 1.0;
+  double* acc = (double*)calloc((size_t)4 * K, sizeof(double)); /* l1, inf, dot, y_l1 */
+#pragma omp parallel
+  {
+    double* a = (double*)calloc((size_t)4 * K, sizeof(double));
+#pragma omp for schedule(dynamic, 1024)
+    for (int32_t i = 0; i < n; ++i)
+      for (int32_t k = 0; k < K; ++k) {
+        const double y = ppr_y(mode, rp, ci, outdeg, i, K, k, NULL, X);
+        const double r = (double)R[(int64_t)i * K + k];
+        const double e = fabs((double)X[(int64_t)i * K + k] - (alpha * y + c * r));
+        a[k] += e;
+        if (e > a[K + k]) a[K + k] = e;
+        if (L) {
+          a[2 * K + k] += (double)L[(int64_t)i * K + k] * (y - r);
+          a[3 * K + k] += fabs(y - r);
+        }
+      }
+#pragma omp critical
+    for (int32_t k = 0; k < K; ++k) {
+      acc[k] += a[k];
+      if (a[K + k] > acc[K + k]) acc[K + k] = a[K + k];
+      acc[2 * K + k] += a[2 * K + k];
+      acc[3 * K + k] += a[3 * K + k];
+    }
+    free(a);
+  }
+  for (int32_t k = 0; k < K; ++k) {
+    if (res_l1) res_l1[k] = acc[k];
+    if (res_inf) res_inf[k] = acc[K + k];
+    if (dot) dot[k] = acc[2 * K + k];
+    if (y_l1) y_l1[k] = acc[3 * K + k];
+  }
+  free(acc);
+}

@@ -7,8 +7,10 @@
  * The product (graph-bit-ops_amd/) never calls it.
  *
  * Every function names the reference file:line whose semantics it restates
- * (paths relative to the reference tree).  Pinning (what each function is
- * checked against besides the GPU) is listed in oracle/README.md.
+ * (paths relative to the reference tree).  What pins each function besides
+ * the GPU (dense brute force, closed forms, textbook algorithms, library
+ * results, finite differences, distribution checks) is in
+ * tests/test_oracle.py.
  *
  * Layout: tiles are stored as T*bs words of bs bits (uint8/16/32/64 for bs
  * 8/16/32/64), word r of tile t at index t*bs + r, bit (bs-1-c) = column c
@@ -90,6 +92,28 @@ void or_dijkstra(int32_t n, const int32_t* rp, const int32_t* ci, const uint32_t
  * until the L1 change is below tol or max_iter; returns iterations. */
 int32_t or_pagerank(int32_t n, const int32_t* rp, const int32_t* ci, const int32_t* outdeg, double alpha,
                     double tol, int32_t max_iter, double* p);
+/* Personalized PageRank in FP64, K columns in vertex-major [n][K] layout.
+ * mode 0, the forward map of algorithm/pr.cu:449-516 with a personalization
+ * vector R (pull CSR: row i lists the in-neighbours of i):
+ *   T(X)[i][k] = alpha * sum_{j in row i} X[j][k] / outdeg[j] + (1 - alpha) R[i][k]
+ * mode 1, its adjoint (out-edge CSR: row i lists the out-neighbours of i):
+ *   T(X)[i][k] = alpha / outdeg[i] * sum_{j in row i} X[j][k] + R[i][k]
+ * (the sum term is 0 where outdeg is 0).  Both maps are alpha-contractions
+ * (mode 0 in the 1-norm, mode 1 in the max-norm), so for any X the distance
+ * to the fixed point is at most |X - T(X)| / (1 - alpha) in that norm.
+ * or_ppr_solve iterates X <- T(X) from the X given until the largest column
+ * change relative to that column's |R|_1 is below tol; returns the number
+ * of iterations, or -1 if max_iter is reached first. */
+int32_t or_ppr_solve(int32_t n, const int32_t* rp, const int32_t* ci, const int32_t* outdeg, double alpha, int32_t K,
+                     int mode, const double* R, double* X, double tol, int32_t max_iter);
+/* One application of T to FP32 X, in FP64: per column the 1-norm and the
+ * max-norm of X - T(X).  With L (mode 0 only) also, per column,
+ * dot[k] = sum_i L[i][k] (y[i][k] - R[i][k]) and y_l1[k] = |y_k - R_k|_1,
+ * where y = sum_{j in row i} X[j][k] / outdeg[j] (the teleport-factor
+ * gradient's product).  Arrays that are not wanted may be NULL. */
+void or_ppr_check(int32_t n, const int32_t* rp, const int32_t* ci, const int32_t* outdeg, double alpha, int32_t K,
+                  int mode, const float* X, const float* R, const float* L, double* res_l1, double* res_inf,
+                  double* dot, double* y_l1);
 /* Union-find connected components, label = smallest vertex id. */
 void or_cc(int32_t n, const int32_t* rp, const int32_t* ci, int32_t* label);
 /* Triangles of a symmetric simple graph by sorted-list intersection on

@@ -51,6 +51,8 @@ def lib():
             "or_bfs_queue": ([c_int32, P, P, c_int32, P], c_int32),
             "or_dijkstra": ([c_int32, P, P, P, c_int32, P], None),
             "or_pagerank": ([c_int32, P, P, P, c_double, c_double, c_int32, P], c_int32),
+            "or_ppr_solve": ([c_int32, P, P, P, c_double, c_int32, c_int, P, P, c_double, c_int32], c_int32),
+            "or_ppr_check": ([c_int32, P, P, P, c_double, c_int32, c_int, P, P, P, P, P, P, P], None),
             "or_cc": ([c_int32, P, P, P], None),
             "or_tc": ([c_int32, P, P], c_uint64),
         }
@@ -216,6 +218,31 @@ def pagerank(n, rp, ci, outdeg, alpha=0.85, tol=1e-13, max_iter=10000):
     it = lib().or_pagerank(n, _p(np.ascontiguousarray(rp, np.int32)), _p(np.ascontiguousarray(ci, np.int32)),
                            _p(np.ascontiguousarray(outdeg, np.int32)), alpha, tol, max_iter, _p(p))
     return p, it
+
+
+def ppr_solve(n, rp, ci, outdeg, alpha, mode, R, X0=None, tol=1e-13, max_iter=10000):
+    """FP64 fixed point of the PPR map (mode 0 forward over the pull CSR,
+    mode 1 adjoint over the out-edge CSR); R and the result are [n][K]."""
+    R = np.ascontiguousarray(R, np.float64)
+    K = R.shape[1]
+    X = np.ascontiguousarray(R if X0 is None else X0, np.float64).copy()
+    it = lib().or_ppr_solve(n, _p(np.ascontiguousarray(rp, np.int32)), _p(np.ascontiguousarray(ci, np.int32)),
+                            _p(np.ascontiguousarray(outdeg, np.int32)), alpha, K, mode, _p(R), _p(X), tol, max_iter)
+    return X, it
+
+
+def ppr_check(n, rp, ci, outdeg, alpha, mode, X, R, L=None):
+    """Per column: (|X - T(X)|_1, |X - T(X)|_inf, dot, y_l1) in FP64 for FP32
+    X, R (and L): see oracle.h."""
+    X = np.ascontiguousarray(X, np.float32)
+    R = np.ascontiguousarray(R, np.float32)
+    K = X.shape[1]
+    out = [np.zeros(K) for _ in range(4)]
+    Lp = None if L is None else np.ascontiguousarray(L, np.float32)
+    lib().or_ppr_check(n, _p(np.ascontiguousarray(rp, np.int32)), _p(np.ascontiguousarray(ci, np.int32)),
+                       _p(np.ascontiguousarray(outdeg, np.int32)), alpha, K, mode, _p(X), _p(R), _p(Lp),
+                       *[_p(o) for o in out])
+    return tuple(out)
 
 
 def cc(n, rp, ci):

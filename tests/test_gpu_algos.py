@@ -167,6 +167,15 @@ def test_driver_results_in_device_memory(gbo, po, bs):
     assert np.array_equal(d.to_numpy(np.float32, n).view(np.uint32), p_host.view(np.uint32))
     with pytest.raises(ValueError):
         gbo.cc(A, out=gbo.DeviceBuffer(4 * n - 4))
+    # multi-source levels are [nsrc][n]: a buffer for one source fewer is refused
+    src = np.array([0, 5, 9], np.int32)
+    with pytest.raises(ValueError):
+        gbo.bfs_multisource(A, src, out=gbo.DeviceBuffer(4 * n * 2))
+    dm = gbo.DeviceBuffer(4 * n * 3)
+    assert gbo.bfs_multisource(A, src, out=dm) >= 1
+    lvm = dm.to_numpy(np.int32, 3 * n).reshape(3, n)
+    for k, s in enumerate(src):
+        assert np.array_equal(lvm[k], po.bfs(n, g.rowptr, g.colind, int(s)))
 
 
 @pytest.mark.parametrize("bs", BS)
@@ -346,6 +355,19 @@ def test_graph_batch_barrier_timeout(gbo, po, monkeypatch):
         B.run(src, use_graph=False)
     monkeypatch.delenv("GBO_BATCH_SPIN_MS")
     B.run(src, use_graph=False)
+    levels, labels = B.results()
+    for i, g in enumerate(graphs):
+        assert np.array_equal(levels[i], po.bfs(g.nrows, g.rowptr, g.colind, 0)), i
+        assert np.array_equal(labels[i], po.cc(g.nrows, g.rowptr, g.colind)), i
+    # the captured launch: a graph captured with the normal limit is captured
+    # again when the limit changes (the error names the limit that applied),
+    # and once more when it changes back
+    B.run(src, use_graph=True)
+    monkeypatch.setenv("GBO_BATCH_SPIN_MS", "0")
+    with pytest.raises(gbo.GboError, match="exceeded 0 ms"):
+        B.run(src, use_graph=True)
+    monkeypatch.delenv("GBO_BATCH_SPIN_MS")
+    B.run(src, use_graph=True)
     levels, labels = B.results()
     for i, g in enumerate(graphs):
         assert np.array_equal(levels[i], po.bfs(g.nrows, g.rowptr, g.colind, 0)), i

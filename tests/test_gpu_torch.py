@@ -1,0 +1,89 @@
+"""The torch.autograd.Function on GPU tensors: forward and backward keep
+every [n][K] array on the device (no host copy of S, P, W or grad_S), and
+give the same bits as the host-array path.
+
+It runs in a child process that imports torch before the engine, so that
+libgbo.so binds to torch's HIP runtime (one runtime in the process: device
+addresses can be passed between them)."""
+import os
+import subprocess
+import sys
+import textwrap
+
+import pytest
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+CHILD = textwrap.dedent(r"""
+    import os, sys
+    import torch  # first: libgbo.so then binds to torch's HIP runtime
+    root = sys.argv[1]
+    sys.path[:0] = [os.path.join(root, "graph-bit-ops_amd"), os.path.join(root, "oracle")]
+    import numpy as np
+    import gbo, gbo_torch
+    import ppr_oracle as pp
+    print("case: R-MAT s9 ef8 seed 5, K=6, bs 8 and 16", flush=True)
+    g = gbo.gen_rmat(9, 8, seed=5)
+    rng = np.random.default_rng(5)
+    n, K = g.nrows, 6
+    S = np.zeros((n, K))
+    for k in range(K):
+        idx = rng.choice(n, 20, replace=False)
+        S[idx, k] = rng.dirichlet(np.full(20, 0.1))
+    W = rng.standard_normal((n, K))
+    T = pp.transition(n, g.rowptr, g.colind)
+    wS, wa = pp.adjoint(T, S, W, 0.85)
+    assert len(gbo.hip_runtimes()) == 1, gbo.hip_runtimes()
+    for bs in (8, 16):
+        A = gbo.BsrMatrix.from_csr(g, bs)
+        # host-array path
+        Sh = torch.tensor(S, dtype=torch.float32, requires_grad=True)
+        ah = torch.tensor(0.85, dtype=torch.float32, requires_grad=True)
+        Ph = gbo_torch.personalized_pagerank(Sh, ah, A, A, g.outdeg(), 1e-9, 2000)
+        (Ph * torch.tensor(W, dtype=torch.float32)).sum().backward()
+        # device path, with every host copy of the engine and of tensors refused
+        dev = torch.device("cuda:0")
+        Sd = torch.tensor(S, dtype=torch.float32, device=dev, requires_grad=True)
+        ad = torch.tensor(0.85, dtype=torch.float32, device=dev, requires_grad=True)
+        Wd = torch.tensor(W, dtype=torch.float32, device=dev)
+        od = torch.tensor(g.outdeg(), dtype=torch.int32, device=dev)
+        torch.cuda.synchronize()
+        def refuse(*a, **k):
+            raise AssertionError("host copy on the device path")
+        saved = (torch.Tensor.cpu, torch.Tensor.numpy, gbo.DeviceBuffer.from_numpy, gbo.DeviceBuffer.to_numpy,
+                 gbo.ppr_forward, gbo.ppr_adjoint)
+        torch.Tensor.cpu = torch.Tensor.numpy = refuse
+        gbo.DeviceBuffer.from_numpy = gbo.DeviceBuffer.to_numpy = refuse
+        gbo.ppr_forward = gbo.ppr_adjoint = refuse
+        try:
+            Pd = gbo_torch.personalized_pagerank(Sd, ad, A, A, od, 1e-9, 2000)
+            assert Pd.is_cuda
+            (Pd * Wd).sum().backward()
+        finally:
+            (torch.Tensor.cpu, torch.Tensor.numpy, gbo.DeviceBuffer.from_numpy, gbo.DeviceBuffer.to_numpy,
+             gbo.ppr_forward, gbo.ppr_adjoint) = saved
+        torch.cuda.synchronize()
+        assert Sd.grad.is_cuda and ad.grad.is_cuda
+        assert torch.equal(Pd.cpu(), Ph.detach()), bs
+        assert torch.equal(Sd.grad.cpu(), Sh.grad), bs
+        assert torch.equal(ad.grad.cpu(), ah.grad), bs
+        gs = Sd.grad.cpu().numpy().astype(np.float64)
+        assert np.abs(gs - wS).max() <= 1e-5 * np.abs(wS).max(), bs
+        assert abs(float(ad.grad.cpu()) - wa) <= 1e-5 * abs(wa), bs
+        # an iteration cap below convergence raises (strict) instead of
+        # returning an unconverged iterate
+        try:
+            gbo_torch.personalized_pagerank(Sd, ad, A, A, od, 1e-12, 2)
+            raise SystemExit("no GBO_ENOCONV")
+        except gbo.GboError as e:
+            assert "GBO_ENOCONV" in str(e)
+        print("bs", bs, "ok", flush=True)
+    print("device path ok", flush=True)
+""")
+
+
+def test_ppr_autograd_device_tensors():
+    r = subprocess.run([sys.executable, "-c", CHILD, ROOT], capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stdout[-4000:] + r.stderr[-4000:]
+    assert "device path ok" in r.stdout

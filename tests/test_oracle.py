@@ -237,3 +237,255 @@ def test_ppr_oracle_finite_differences():
     # p is a fixed point of the PageRank update
     p = pp.forward(P, S, alpha)
     assert np.abs(p - (alpha * P.T @ p + (1 - alpha) * S)).max() < 1e-12
+
+
+# ------------------------------------------------------------------------
+# Value products (sum, min, min-plus) and the BFS step, each against a
+# dense brute force built from the CSR directly (not from the tiles).  One
+# case is directed, so a transposed operand (a row/column mix-up in the
+# tile loops) fails; two sizes are not multiples of 64, so the padded last
+# block row and column are covered; one matrix is empty.
+def _value_cases(po):
+    rp, ci = po.gen_er(1024, 0.02, 0)
+    yield "er1024", 1024, rp, ci
+    rp, ci = random_sym(1000, 3000, 7)
+    yield "rand1000", 1000, rp, ci
+    rng = np.random.default_rng(21)
+    u, v = rng.integers(0, 517, 4000), rng.integers(0, 517, 4000)
+    rp, ci = po.csr_from_edges(517, u, v, symmetric=False, drop_loops=False)
+    yield "directed517", 517, rp, ci
+    yield "empty130", 130, np.zeros(131, np.int32), np.zeros(0, np.int32)
+
+
+@pytest.mark.parametrize("bs", BS)
+def test_sum_f64_dense(po, bs):
+    """or_bsrbmv_sum_f64: y = A x in FP64 (signed x, so a dropped or doubled
+    term shows)."""
+    rng = np.random.default_rng(bs)
+    for name, n, rp, ci in _value_cases(po):
+        D = dense(n, rp, ci).astype(np.float64)
+        x = rng.standard_normal(n)
+        pk = po.pack(bs, n, rp, ci)
+        y = po.sum_f64(bs, n, *pk, x)
+        want = D @ x
+        bound = 1e-12 * (np.abs(D) @ np.abs(x)) + 1e-300
+        assert np.all(np.abs(y - want) <= bound), name
+        if name == "directed517":  # the transpose gives a different answer
+            assert not np.allclose(y, D.T @ x)
+
+
+@pytest.mark.parametrize("bs", BS)
+def test_min_i32_dense(po, bs):
+    """or_bsrbmv_min_i32: y[i] = min over the row's neighbours j of x[j],
+    INT32_MAX for a row without neighbours; x spans negative values."""
+    rng = np.random.default_rng(bs + 100)
+    imax = np.iinfo(np.int32).max
+    for name, n, rp, ci in _value_cases(po):
+        D = dense(n, rp, ci).astype(bool)
+        x = rng.integers(-2**31, 2**31 - 1, n, dtype=np.int64).astype(np.int32)
+        x[: n // 3] = rng.integers(-50, 50, n // 3)
+        pk = po.pack(bs, n, rp, ci)
+        y = po.min_i32(bs, n, *pk, x)
+        want = np.where(D, x[None, :].astype(np.int64), imax).min(axis=1) if n else np.zeros(0)
+        assert np.array_equal(y, want.astype(np.int32)), name
+        deg = np.diff(rp)
+        assert np.all(y[deg == 0] == imax), name
+
+
+def _sat_add(a, b):
+    return np.minimum(a.astype(np.int64) + b.astype(np.int64), 0xFFFFFFFF)
+
+
+@pytest.mark.parametrize("bs", BS)
+def test_minplus_unit_dense(po, bs):
+    """Unit-weight branch of or_bsrbmv_minplus (w = NULL): out[i] =
+    min(dist[i], min_j dist[j] + 1) saturating at 0xFFFFFFFF; its fixpoint
+    from a single source equals the BFS levels (scipy)."""
+    rng = np.random.default_rng(bs + 200)
+    for name, n, rp, ci in _value_cases(po):
+        D = dense(n, rp, ci).astype(bool)
+        dist = rng.integers(0, 5000, n).astype(np.uint32)
+        dist[rng.random(n) < 0.3] = 0xFFFFFFFF
+        dist[rng.random(n) < 0.05] = 0xFFFFFFFE  # +1 lands exactly on INF
+        pk = po.pack(bs, n, rp, ci)
+        out = po.minplus(bs, n, *pk, dist)
+        cand = np.where(D, _sat_add(dist, np.ones(n, np.uint32))[None, :], 0xFFFFFFFF).min(axis=1) if n else dist
+        want = np.minimum(dist.astype(np.int64), cand).astype(np.uint32)
+        assert np.array_equal(out, want), name
+        if n == 0 or name == "directed517":
+            continue
+        d = np.full(n, 0xFFFFFFFF, np.uint32)
+        d[0] = 0
+        for _ in range(n + 1):
+            nd = po.minplus(bs, n, *pk, d)
+            if np.array_equal(nd, d):
+                break
+            d = nd
+        A = sp.csr_matrix((np.ones(len(ci)), ci, rp), shape=(n, n))
+        lv = cg.shortest_path(A, unweighted=True, indices=0)
+        assert np.array_equal(d, np.where(np.isinf(lv), 0xFFFFFFFF, lv).astype(np.uint32)), name
+
+
+@pytest.mark.parametrize("bs", (8, 64))
+def test_minplus_weighted_dense(po, gbo, bs):
+    """Weighted branch, one product: out[i] = min(dist[i], min_j dist[j] +
+    W[i,j]) with the weights addressed through the tile bit order (woff +
+    rank); directed, so a weight read from the mirrored edge fails."""
+    rng = np.random.default_rng(bs + 300)
+    for name, n, rp, ci in _value_cases(po):
+        if len(ci) == 0:
+            continue
+        w = rng.integers(1, 65, len(ci)).astype(np.uint32)
+        W = np.zeros((n, n), np.int64)
+        for r in range(n):
+            W[r, ci[rp[r]:rp[r + 1]]] = w[rp[r]:rp[r + 1]]
+        dist = rng.integers(0, 100, n).astype(np.uint32)
+        dist[rng.random(n) < 0.3] = 0xFFFFFFFF
+        pk = po.pack(bs, n, rp, ci)
+        tv, woff = po.pack_values(bs, n, rp, ci, w, *pk)
+        out = po.minplus(bs, n, *pk, dist, tv, woff)
+        cand = np.where(W > 0, np.minimum(dist.astype(np.int64)[None, :] + W, 0xFFFFFFFF), 0xFFFFFFFF).min(axis=1)
+        want = np.minimum(dist.astype(np.int64), cand).astype(np.uint32)
+        assert np.array_equal(out, want), name
+
+
+@pytest.mark.parametrize("bs", BS)
+def test_bfs_step_dense(po, bs):
+    """or_bsrbmv_bfs_step with arbitrary frontier and visited sets (not only
+    those a BFS reaches): next = (A f > 0) & ~visited, count = |next|."""
+    rng = np.random.default_rng(bs + 400)
+    for name, n, rp, ci in _value_cases(po):
+        D = dense(n, rp, ci)
+        f = (rng.random(n) < 0.1).astype(np.float32)
+        vis = (rng.random(n) < 0.4).astype(np.float32)
+        pk = po.pack(bs, n, rp, ci)
+        nx, cnt = po.bfs_step(bs, n, *pk, po.pack_vec(bs, f), po.pack_vec(bs, vis))
+        want = ((D @ f.astype(np.int64)) > 0) & (vis == 0)
+        assert np.array_equal(nx, po.pack_vec(bs, want.astype(np.float32))), name
+        assert cnt == int(want.sum()), name
+
+
+# ------------------------------------------------------------------------
+# Generators: the engine's generators are checked against the oracle's, so
+# the oracle's must be pinned by what the distributions fix.
+def _rmat_raw(po, scale, ef, a, b, c, seed):
+    m = ef << scale
+    u = np.empty(m, np.int32)
+    v = np.empty(m, np.int32)
+    po.lib().or_gen_rmat_edges(scale, ef, a, b, c, seed, u.ctypes.data, v.ctypes.data)
+    return u.astype(np.int64), v.astype(np.int64)
+
+
+@pytest.mark.parametrize("abc", ((0.57, 0.19, 0.19), (0.45, 0.25, 0.20)))
+def test_rmat_quadrant_frequencies(po, abc):
+    """Every recursion level of the raw R-MAT edge list (before
+    symmetrisation) picks quadrant (row half, column half) = (0,0), (0,1),
+    (1,0), (1,1) with probability a, b, c, d: each frequency within 5 sigma.
+    The second parameter set has b != c, so swapping the row and column bit
+    or two quadrants fails."""
+    a, b, c = abc
+    probs = {(0, 0): a, (0, 1): b, (1, 0): c, (1, 1): 1 - a - b - c}
+    scale, ef = 14, 16
+    u, v = _rmat_raw(po, scale, ef, a, b, c, 7)
+    m = len(u)
+    assert u.min() >= 0 and u.max() < 1 << scale and v.max() < 1 << scale
+    for level in range(scale):
+        sh = scale - 1 - level  # level 0 decides the top bit
+        bu, bv = (u >> sh) & 1, (v >> sh) & 1
+        for (qu, qv), p in probs.items():
+            freq = np.count_nonzero((bu == qu) & (bv == qv)) / m
+            assert abs(freq - p) <= 5 * math.sqrt(p * (1 - p) / m), (level, qu, qv, freq, p)
+    # vertex 0 as a source: every level picks the top row half, (a + b)^scale
+    p0 = (a + b) ** scale
+    k0 = np.count_nonzero(u == 0)
+    assert abs(k0 - m * p0) <= 5 * math.sqrt(m * p0 * (1 - p0))
+
+
+def test_er_edge_count_and_uniformity(po):
+    """G(n, p): the number of undirected edges is Binomial(n(n-1)/2, p)
+    (within 5 sigma), every pair u < v, and the degrees have the binomial
+    mean and variance within sampling error."""
+    n, p = 2000, 0.01
+    m = po.lib().or_gen_er_edges(n, p, 3, None, None, 0)
+    pairs = n * (n - 1) // 2
+    assert abs(m - pairs * p) <= 5 * math.sqrt(pairs * p * (1 - p))
+    u = np.empty(m, np.int32)
+    v = np.empty(m, np.int32)
+    po.lib().or_gen_er_edges(n, p, 3, u.ctypes.data, v.ctypes.data, m)
+    assert np.all(u < v) and len(np.unique(u.astype(np.int64) * n + v)) == m
+    deg = np.bincount(np.concatenate([u, v]), minlength=n)
+    mu = (n - 1) * p
+    assert abs(deg.mean() - mu) <= 5 * math.sqrt(mu * (1 - p) / n) * 2
+    assert 0.8 * mu * (1 - p) <= deg.var() <= 1.2 * mu * (1 - p)
+
+
+# ------------------------------------------------------------------------
+# FP64 personalized PageRank (or_ppr_solve / or_ppr_check): the fixed points
+# equal the dense solves of oracle/ppr_oracle.py (themselves pinned by
+# finite differences above), on a directed graph with vertices without
+# out-edges, so a swapped pull/push CSR or a dropped teleport term fails.
+def _directed_case():
+    import pyoracle as po
+    rng = np.random.default_rng(33)
+    n = 90
+    u, v = rng.integers(0, n, 400), rng.integers(0, n, 400)
+    u[u < 6] = 6  # vertices 0..5 have no out-edges
+    rp, ci = po.csr_from_edges(n, u, v, symmetric=False)
+    return n, rp, ci
+
+
+def test_ppr_solve_dense():
+    import ppr_oracle as pp
+    import pyoracle as po
+    n, rp, ci = _directed_case()
+    deg = np.diff(rp).astype(np.int32)
+    T = pp.transition(n, rp, ci)
+    rng = np.random.default_rng(4)
+    S = rng.dirichlet(np.full(n, 0.3), size=3).T
+    W = rng.standard_normal((n, 3))
+    # forward pulls over A^T: row i of the transpose lists i's in-neighbours
+    tr = sp.csr_matrix((np.ones(len(ci)), ci, rp), shape=(n, n)).T.tocsr()
+    trp, tci = tr.indptr.astype(np.int32), tr.indices.astype(np.int32)
+    P, it = po.ppr_solve(n, trp, tci, deg, 0.85, 0, S)
+    assert it > 0 and np.abs(P - pp.forward(T, S, 0.85)).max() < 1e-12
+    lam, it = po.ppr_solve(n, rp, ci, deg, 0.85, 1, W)
+    wl = np.linalg.solve(np.eye(n) - 0.85 * T, W)
+    assert it > 0 and np.abs(lam - wl).max() < 1e-11 * np.abs(wl).max()
+    # the wrong CSR for either map gives a different answer
+    P2, _ = po.ppr_solve(n, rp, ci, deg, 0.85, 0, S)
+    assert np.abs(P2 - P).max() > 1e-3
+
+
+def test_ppr_check_bounds():
+    """or_ppr_check: the residual of the exact fixed point is at rounding
+    level; for a perturbed X the residual bound |X - T(X)|/(1-alpha) covers
+    the true distance (1-norm for the forward, max-norm for the adjoint);
+    dot and y_l1 equal lambda^T (P^T p - s) and |P^T p - s|_1 computed
+    densely."""
+    import ppr_oracle as pp
+    import pyoracle as po
+    n, rp, ci = _directed_case()
+    deg = np.diff(rp).astype(np.int32)
+    T = pp.transition(n, rp, ci)
+    tr = sp.csr_matrix((np.ones(len(ci)), ci, rp), shape=(n, n)).T.tocsr()
+    trp, tci = tr.indptr.astype(np.int32), tr.indices.astype(np.int32)
+    rng = np.random.default_rng(5)
+    S = rng.dirichlet(np.full(n, 0.3), size=2).T
+    W = rng.standard_normal((n, 2))
+    a = 0.85
+    P = pp.forward(T, S, a)
+    lam = np.linalg.solve(np.eye(n) - a * T, W)
+    l1, linf, dot, yl1 = po.ppr_check(n, trp, tci, deg, a, 0, P, S, lam)
+    assert l1.max() < 1e-6  # FP32 rounding of P only
+    y = T.T @ P.astype(np.float32).astype(np.float64)
+    lam32 = lam.astype(np.float32).astype(np.float64)
+    np.testing.assert_allclose(dot, (lam32 * (y - S.astype(np.float32))).sum(0), rtol=1e-9)
+    np.testing.assert_allclose(yl1, np.abs(y - S.astype(np.float32)).sum(0), rtol=1e-9)
+    Xp = P + 1e-3 * rng.standard_normal(P.shape)
+    l1, _, _, _ = po.ppr_check(n, trp, tci, deg, a, 0, Xp, S)
+    true = np.abs(Xp.astype(np.float32) - P).sum(0)
+    assert np.all(true <= l1 / (1 - a) + 1e-6) and np.all(l1 > 0.1 * true)
+    Lp = lam + 1e-3 * rng.standard_normal(lam.shape)
+    _, linf, _, _ = po.ppr_check(n, rp, ci, deg, a, 1, Lp, W)
+    true = np.abs(Lp.astype(np.float32) - lam).max(0)
+    assert np.all(true <= linf / (1 - a) + 1e-5)
