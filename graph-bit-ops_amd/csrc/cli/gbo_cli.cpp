@@ -19,6 +19,9 @@
 //   --out FILE    write the per-vertex result (levels, distances, scores,
 //                 labels) one value per line
 //   --json        print one machine-readable JSON result line at the end
+//   --cache       read through the binary CSR cache beside the .mtx file
+//                 (readMtx's ".<file>.<ud|d>.<nosl|sl>.bin"); also on when
+//                 the environment variable GBO_MTX_CACHE is 1
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -43,6 +46,7 @@ struct Opts {
   double tol = 1e-9;
   std::string out;
   bool json = false;
+  bool cache = false;
 };
 
 [[noreturn]] void die(const char* what, int rc) {
@@ -62,7 +66,7 @@ double now_ms() {
 void usage() {
   std::fprintf(stderr,
                "usage: gbo <bsrbmv|bsrbmm|tc|bfs|sssp|pr|cc> [--bs N] [--seed S] [--source V] [--max-iter K]\n"
-               "           [--tol T] [--out FILE] [--json] file.mtx [directed]\n"
+               "           [--tol T] [--out FILE] [--json] [--cache] file.mtx [directed]\n"
                "   or: bsrbmv32 file.mtx, bfs8 file.mtx 0, ... (name carries command and block size)\n");
   std::exit(2);
 }
@@ -80,7 +84,19 @@ struct HostCsr {
 
 void read_mtx(const Opts& o, int semantics, HostCsr* c) {
   std::printf("input sparse matrix: %s\n", o.file.c_str());
-  CK(gbo_mtx_read(o.file.c_str(), semantics, o.directed, &c->nrows, &c->ncols, &c->nnz, &c->rp, &c->ci, nullptr));
+  const char* env = std::getenv("GBO_MTX_CACHE");
+  const double t0 = now_ms();
+  if (o.cache || (env && std::atoi(env) == 1)) {
+    int state = -1;
+    CK(gbo_mtx_read_cached(o.file.c_str(), semantics, o.directed, &state, &c->nrows, &c->ncols, &c->nnz, &c->rp,
+                           &c->ci, nullptr));
+    char name[4096];
+    CK(gbo_mtx_cache_path(o.file.c_str(), semantics, o.directed, name, sizeof(name)));
+    std::printf("csr cache: %s (%s)\n", name, state == 1 ? "loaded" : state == 0 ? "written" : "not writable");
+  } else {
+    CK(gbo_mtx_read(o.file.c_str(), semantics, o.directed, &c->nrows, &c->ncols, &c->nnz, &c->rp, &c->ci, nullptr));
+  }
+  std::printf("read time: %.1f ms\n", now_ms() - t0);
   std::printf("nrows: %d, ncols: %d, nnz: %lld\n", c->nrows, c->ncols, (long long)c->nnz);
 }
 
@@ -365,6 +381,7 @@ int main(int argc, char** argv) {
     else if (a == "--tol") o.tol = std::atof(val());
     else if (a == "--out") o.out = val();
     else if (a == "--json") o.json = true;
+    else if (a == "--cache") o.cache = true;
     else if (a == "-h" || a == "--help") usage();
     else pos.push_back(a);
   }

@@ -106,84 +106,6 @@ static int build_symmetric(int32_t n, int64_t m, EdgeFn edge, int64_t* nnz, int3
   return GBO_OK;
 }
 
-// ------------------------------------------------------------ Matrix Market
-struct MtxHeader {
-  bool coordinate = false;
-  char field = 'R';     // R real, C complex, I integer, P pattern
-  char symmetry = 'G';  // G general, S symmetric, H hermitian, K skew
-  int64_t m = 0, n = 0, nnz = 0;
-};
-
-static std::string lower(std::string s) {
-  for (auto& ch : s) ch = (char)std::tolower((unsigned char)ch);
-  return s;
-}
-
-// Parses the banner and size line (mm_read_banner / mm_read_mtx_crd_size).
-static int read_header(FILE* f, MtxHeader* h) {
-  char line[1030];
-  if (!std::fgets(line, sizeof(line), f)) return set_error(GBO_EFORMAT, "empty file");
-  char b0[64] = {0}, b1[64] = {0}, b2[64] = {0}, b3[64] = {0}, b4[64] = {0};
-  if (std::sscanf(line, "%63s %63s %63s %63s %63s", b0, b1, b2, b3, b4) != 5)
-    return set_error(GBO_EFORMAT, "could not process Matrix Market banner");
-  if (std::string(b0) != "%%MatrixMarket" || lower(b1) != "matrix")
-    return set_error(GBO_EFORMAT, "could not process Matrix Market banner");
-  const std::string fmt = lower(b2), fld = lower(b3), sym = lower(b4);
-  if (fmt != "coordinate") return set_error(GBO_EFORMAT, "only coordinate format is supported");
-  h->coordinate = true;
-  if (fld == "real") h->field = 'R';
-  else if (fld == "complex") h->field = 'C';
-  else if (fld == "integer") h->field = 'I';
-  else if (fld == "pattern") h->field = 'P';
-  else return set_error(GBO_EFORMAT, "unknown field type '%s'", b3);
-  if (sym == "general") h->symmetry = 'G';
-  else if (sym == "symmetric") h->symmetry = 'S';
-  else if (sym == "hermitian") h->symmetry = 'H';
-  else if (sym == "skew-symmetric") h->symmetry = 'K';
-  else return set_error(GBO_EFORMAT, "unknown symmetry '%s'", b4);
-  // Comments, then the size line.
-  while (true) {
-    if (!std::fgets(line, sizeof(line), f)) return set_error(GBO_EFORMAT, "missing size line");
-    const char* p = line;
-    while (*p == ' ' || *p == '\t') ++p;
-    if (*p == '%' || *p == '\n' || *p == '\r' || *p == 0) continue;
-    long long m, n, nz;
-    if (std::sscanf(p, "%lld %lld %lld", &m, &n, &nz) != 3)
-      return set_error(GBO_EFORMAT, "malformed size line");
-    if (m < 0 || n < 0 || nz < 0 || m >= (1LL << 31) || n >= (1LL << 31) || nz >= (1LL << 31))
-      return set_error(GBO_ERANGE, "matrix size out of range");
-    h->m = m;
-    h->n = n;
-    h->nnz = nz;
-    return GBO_OK;
-  }
-}
-
-struct Triple {
-  int32_t r, c;
-  float v;
-};
-
-static int read_entries(FILE* f, const MtxHeader& h, std::vector<Triple>* out) {
-  out->resize((size_t)h.nnz);
-  for (int64_t i = 0; i < h.nnz; ++i) {
-    long long r, c;
-    if (std::fscanf(f, "%lld %lld", &r, &c) != 2)
-      return set_error(GBO_EFORMAT, "not enough entries in mtx file (%lld of %lld)", (long long)i,
-                       (long long)h.nnz);
-    double v = 1.0, vi = 0.0;
-    if (h.field == 'R' || h.field == 'I') {
-      if (std::fscanf(f, "%lf", &v) != 1) return set_error(GBO_EFORMAT, "missing value");
-    } else if (h.field == 'C') {
-      if (std::fscanf(f, "%lf %lf", &v, &vi) != 2) return set_error(GBO_EFORMAT, "missing value");
-    }
-    if (r < 1 || r > h.m || c < 1 || c > h.n)
-      return set_error(GBO_EFORMAT, "entry %lld (%lld, %lld) out of bounds", (long long)i, r, c);
-    (*out)[(size_t)i] = Triple{(int32_t)(r - 1), (int32_t)(c - 1), (float)v};
-  }
-  return GBO_OK;
-}
-
 }  // namespace gbo
 
 using namespace gbo;
@@ -193,130 +115,6 @@ extern "C" {
 const char* gbo_last_error(void) { return g_last_error.c_str(); }
 const char* gbo_version(void) { return "gbo 0.1 (gfx950)"; }
 void gbo_free_host(void* p) { std::free(p); }
-
-int gbo_mtx_read(const char* path, int semantics, int directed, int32_t* nrows, int32_t* ncols,
-                 int64_t* nnz, int32_t** h_rowptr, int32_t** h_colind, float** h_val) {
-  GBO_TRY_BEGIN
-  if (!path || !nrows || !ncols || !nnz || !h_rowptr || !h_colind)
-    return set_error(GBO_EINVAL, "gbo_mtx_read: null argument");
-  if (semantics != 0 && semantics != 1)
-    return set_error(GBO_EINVAL, "gbo_mtx_read: semantics must be 0 or 1");
-  if (directed < 0 || directed > 2)
-    return set_error(GBO_EINVAL, "gbo_mtx_read: directed must be 0, 1 or 2");
-  FILE* f = std::fopen(path, "r");
-  if (!f) return set_error(GBO_EIO, "file %s not found", path);
-  MtxHeader h;
-  int rc = read_header(f, &h);
-  std::vector<Triple> e;
-  if (rc == GBO_OK) rc = read_entries(f, h, &e);
-  std::fclose(f);
-  if (rc != GBO_OK) return rc;
-  const int32_t m = (int32_t)h.m, n = (int32_t)h.n;
-
-  if (semantics == 0) {
-    // mmio_data: counting pass (+ mirrored counts), exclusive scan, then
-    // placement in file order; a mirrored entry lands right after its source.
-    const bool sym = (h.symmetry == 'S' || h.symmetry == 'H');
-    std::vector<int64_t> cnt((size_t)m + 1, 0);
-    for (const auto& t : e) {
-      cnt[t.r]++;
-      if (sym && t.r != t.c) {
-        if (t.c >= m) return set_error(GBO_EFORMAT, "symmetric matrix is not square");
-        cnt[t.c]++;
-      }
-    }
-    int64_t total = 0;
-    for (int32_t i = 0; i < m; ++i) total += cnt[i];
-    if (total >= ((int64_t)1 << 31)) return set_error(GBO_ERANGE, "nnz exceeds int32 CSR");
-    int32_t* rp = host_alloc<int32_t>((size_t)m + 1);
-    int32_t* ci = host_alloc<int32_t>((size_t)total);
-    float* cv = h_val ? host_alloc<float>((size_t)total) : nullptr;
-    int64_t acc = 0;
-    for (int32_t i = 0; i < m; ++i) {
-      rp[i] = (int32_t)acc;
-      acc += cnt[i];
-      cnt[i] = rp[i];
-    }
-    rp[m] = (int32_t)acc;
-    for (const auto& t : e) {
-      int64_t o = cnt[t.r]++;
-      ci[o] = t.c;
-      if (cv) cv[o] = t.v;
-      if (sym && t.r != t.c) {
-        o = cnt[t.c]++;
-        ci[o] = t.r;
-        if (cv) cv[o] = t.v;
-      }
-    }
-    *nrows = m;
-    *ncols = n;
-    *nnz = total;
-    *h_rowptr = rp;
-    *h_colind = ci;
-    if (h_val) *h_val = cv;
-    return GBO_OK;
-  }
-
-  // readMtx: undirected when the banner says symmetric (and directed != 1)
-  // or when directed == 2; then mirror off-diagonal entries, sort by
-  // (row, col), drop repeats and (unless GRB_UTIL_REMOVE_SELFLOOP=0)
-  // self-loops.  The first of a run of equal (row, col) keeps its value.
-  bool undirected = (h.symmetry == 'S') || directed == 2;
-  if (directed == 1) undirected = false;
-  const char* env = std::getenv("GRB_UTIL_REMOVE_SELFLOOP");
-  const bool drop_loops = env ? (std::atoi(env) != 0) : true;
-  if (undirected) {
-    const size_t n0 = e.size();
-    for (size_t i = 0; i < n0; ++i)
-      if (e[i].r != e[i].c) {
-        if (e[i].c >= m || e[i].r >= n)
-          return set_error(GBO_EFORMAT, "undirected matrix is not square");
-        e.push_back(Triple{e[i].c, e[i].r, e[i].v});
-      }
-  }
-  std::stable_sort(e.begin(), e.end(), [](const Triple& a, const Triple& b) {
-    return a.r != b.r ? a.r < b.r : a.c < b.c;
-  });
-  std::vector<Triple> k;
-  k.reserve(e.size());
-  for (size_t i = 0; i < e.size(); ++i) {
-    if (i > 0 && e[i].r == e[i - 1].r && e[i].c == e[i - 1].c) continue;
-    if (drop_loops && e[i].r == e[i].c) continue;
-    k.push_back(e[i]);
-  }
-  if (k.size() >= ((size_t)1 << 31)) return set_error(GBO_ERANGE, "nnz exceeds int32 CSR");
-  int32_t* rp = host_alloc<int32_t>((size_t)m + 1);
-  int32_t* ci = host_alloc<int32_t>(k.size());
-  float* cv = h_val ? host_alloc<float>(k.size()) : nullptr;
-  std::fill(rp, rp + m + 1, 0);
-  for (const auto& t : k) rp[t.r + 1]++;
-  for (int32_t i = 0; i < m; ++i) rp[i + 1] += rp[i];
-  for (size_t i = 0; i < k.size(); ++i) {
-    ci[i] = k[i].c;
-    if (cv) cv[i] = k[i].v;
-  }
-  *nrows = m;
-  *ncols = n;
-  *nnz = (int64_t)k.size();
-  *h_rowptr = rp;
-  *h_colind = ci;
-  if (h_val) *h_val = cv;
-  return GBO_OK;
-  GBO_TRY_END
-}
-
-int gbo_mtx_write(const char* path, int32_t nrows, int32_t ncols, const int32_t* rp,
-                  const int32_t* ci) {
-  if (!path || !rp || (!ci && rp[nrows] > 0)) return set_error(GBO_EINVAL, "gbo_mtx_write: null");
-  FILE* f = std::fopen(path, "w");
-  if (!f) return set_error(GBO_EIO, "cannot open %s for writing", path);
-  std::fprintf(f, "%%%%MatrixMarket matrix coordinate pattern general\n%d %d %d\n", nrows, ncols,
-               rp[nrows]);
-  for (int32_t r = 0; r < nrows; ++r)
-    for (int32_t k = rp[r]; k < rp[r + 1]; ++k) std::fprintf(f, "%d %d\n", r + 1, ci[k] + 1);
-  std::fclose(f);
-  return GBO_OK;
-}
 
 // ------------------------------------------------------------- generators
 // Erdos-Renyi G(n, p): unordered pair {u < v} is an edge iff the 53-bit draw

@@ -97,6 +97,9 @@ def _declare(L):
         "gbo_free_host": ([P], None),
         "gbo_stream_copy": ([P, P, c_size_t, P], c_int),
         "gbo_mtx_read": ([c_char_p, c_int, c_int, i32p, i32p, i64p, POINTER(P), POINTER(P), POINTER(P)], c_int),
+        "gbo_mtx_read_cached": ([c_char_p, c_int, c_int, POINTER(c_int), i32p, i32p, i64p, POINTER(P), POINTER(P),
+                                 POINTER(P)], c_int),
+        "gbo_mtx_cache_path": ([c_char_p, c_int, c_int, c_char_p, c_size_t], c_int),
         "gbo_mtx_write": ([c_char_p, c_int32, c_int32, P, P], c_int),
         "gbo_gen_er": ([c_int32, c_double, c_uint64, i64p, POINTER(P), POINTER(P)], c_int),
         "gbo_gen_er_m": ([c_int32, c_int64, c_uint64, i64p, POINTER(P), POINTER(P)], c_int),
@@ -297,19 +300,38 @@ def _take(p_rowptr, nrp, p_colind, nnz):
     return rp, ci
 
 
-def mtx_read(path: str, semantics: int = 1, directed: int = 0, values: bool = False) -> Csr:
-    """Read a Matrix Market file. semantics 0 = mmio_highlevel.h, 1 = readMtx.hpp."""
+def mtx_read(path: str, semantics: int = 1, directed: int = 0, values: bool = False,
+             cache: bool = False) -> Csr:
+    """Read a Matrix Market file. semantics 0 = mmio_highlevel.h, 1 = readMtx.hpp.
+    cache=True goes through gbo_mtx_read_cached (binary CSR cache beside the
+    file); the returned Csr then carries `cache_state` (1 hit, 0 written,
+    2 not writable)."""
     nr, nc, nnz = c_int32(), c_int32(), c_int64()
     rp, ci, val = c_void_p(), c_void_p(), c_void_p()
-    _check(lib().gbo_mtx_read(os.fsencode(path), semantics, directed, byref(nr), byref(nc), byref(nnz),
-                              byref(rp), byref(ci), byref(val) if values else None))
+    state = c_int(-1)
+    if cache:
+        _check(lib().gbo_mtx_read_cached(os.fsencode(path), semantics, directed, byref(state), byref(nr), byref(nc),
+                                         byref(nnz), byref(rp), byref(ci), byref(val) if values else None))
+    else:
+        _check(lib().gbo_mtx_read(os.fsencode(path), semantics, directed, byref(nr), byref(nc), byref(nnz),
+                                  byref(rp), byref(ci), byref(val) if values else None))
     vals = None
     if values:
         vals = (np.ctypeslib.as_array(ctypes.cast(val, POINTER(c_float)), shape=(nnz.value,)).copy()
                 if nnz.value > 0 else np.zeros(0, np.float32))
         lib().gbo_free_host(val)
     rpa, cia = _take(rp, nr.value + 1, ci, nnz.value)
-    return Csr(nr.value, nc.value, rpa, cia, vals)
+    out = Csr(nr.value, nc.value, rpa, cia, vals)
+    if cache:
+        out.cache_state = state.value
+    return out
+
+
+def mtx_cache_path(path: str, semantics: int = 1, directed: int = 0) -> str:
+    """The binary cache file name mtx_read(cache=True) uses."""
+    buf = ctypes.create_string_buffer(4096)
+    _check(lib().gbo_mtx_cache_path(os.fsencode(path), semantics, directed, buf, len(buf)))
+    return os.fsdecode(buf.value)
 
 
 def mtx_write(path: str, csr: Csr):

@@ -111,11 +111,30 @@ int gbo_stream_copy(void* d_dst, const void* d_src, size_t bytes, gbo_stream_t s
  *   (row, col), drop duplicates, drop self-loops unless the environment
  *   variable GRB_UTIL_REMOVE_SELFLOOP is "0" (readMtx.hpp:151-216).
  * Output arrays are allocated by the library; release with gbo_free_host.
- * h_val may be NULL when the values are not wanted. */
+ * h_val may be NULL when the values are not wanted.  The file is read whole
+ * and parsed on all host threads (token-parallel; the result is the same
+ * as the reference's sequential fscanf loops: tokens pair into entries
+ * across lines, tokens past the header's entry count are ignored). */
 int gbo_mtx_read(const char* path, int semantics, int directed,
                  int32_t* nrows, int32_t* ncols, int64_t* nnz,
                  int32_t** h_rowptr, int32_t** h_colind, float** h_val);
-/* Writes a pattern general .mtx file of a CSR (1-based). */
+/* gbo_mtx_read with a binary CSR cache beside the file, the cache that
+ * readMtx names but never writes (algorithm/readMtx.hpp:227-244,390-395):
+ * semantics 1 uses readMtx's own name "<dir>/.<file>.<ud|d>.<nosl|sl>.bin",
+ * semantics 0 "<dir>/.<file>.mmio.bin".  The cache records the source's size
+ * and modification time, the semantics, the undirected flag and the
+ * self-loop rule; a cache that does not match all of them is ignored and
+ * rewritten (through a temporary file renamed into place).  *cache_state
+ * (may be NULL): 1 = loaded from the cache, 0 = parsed and cache written,
+ * 2 = parsed, cache not writable (not an error). */
+int gbo_mtx_read_cached(const char* path, int semantics, int directed, int* cache_state,
+                        int32_t* nrows, int32_t* ncols, int64_t* nnz,
+                        int32_t** h_rowptr, int32_t** h_colind, float** h_val);
+/* The cache file name gbo_mtx_read_cached uses for (path, semantics,
+ * directed), NUL-terminated into buf[len]. */
+int gbo_mtx_cache_path(const char* path, int semantics, int directed, char* buf, size_t len);
+/* Writes a pattern general .mtx file of a CSR (1-based); rows are formatted
+ * on all host threads. */
 int gbo_mtx_write(const char* path, int32_t nrows, int32_t ncols,
                   const int32_t* h_rowptr, const int32_t* h_colind);
 

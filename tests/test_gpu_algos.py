@@ -296,7 +296,9 @@ def test_ppr_autograd(gbo):
     A = gbo.BsrMatrix.from_csr(g, 16)
     St = torch.tensor(S, dtype=torch.float32, requires_grad=True)
     at = torch.tensor(0.85, dtype=torch.float32, requires_grad=True)
-    P = gbo_torch.personalized_pagerank(St, at, A, A, g.outdeg(), 1e-9, 2000)
+    # the wrapper solves strictly: the tolerance must be reachable in FP32
+    # (this case's iterates stop changing near 4e-9 relative L1)
+    P = gbo_torch.personalized_pagerank(St, at, A, A, g.outdeg(), 2e-8, 2000)
     loss = (P * torch.tensor(W, dtype=torch.float32)).sum()
     loss.backward()
     T = pp.transition(g.nrows, g.rowptr, g.colind)

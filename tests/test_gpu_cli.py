@@ -73,3 +73,18 @@ def test_cli_usage_errors():
     assert p.returncode == 1 and "./exe [xxx.mtx]" in p.stdout
     p = subprocess.run([os.path.join(BIN, "bfs32"), "/nonexistent.mtx", "0"], capture_output=True, text=True)
     assert p.returncode != 0 and "not found" in p.stderr
+
+
+def test_cli_csr_cache(gbo, po, tmp_path):
+    """--cache: the first run parses and writes readMtx's cache file, the
+    second loads it; both give the oracle's BFS levels."""
+    import shutil
+    path = str(tmp_path / "g.mtx")
+    shutil.copy(os.path.join(GOLDEN, "lower1000.mtx"), path)
+    g = gbo.mtx_read(path, 1, 0)
+    want = po.bfs(g.nrows, g.rowptr, g.colind, 0)
+    for state in ("written", "loaded"):
+        outf = str(tmp_path / f"lv_{state}.txt")
+        out = run("bfs16", path, "0", "--cache", "--out", outf)
+        assert f"csr cache: {tmp_path}/.g.mtx.ud.nosl.bin ({state})" in out
+        assert np.array_equal(np.loadtxt(outf, dtype=np.int64), want)

@@ -107,7 +107,9 @@ def test_rmat22_drivers(gbo, po, s22, bs):
     lab, it = gbo.cc(A)
     assert np.array_equal(lab, s22.oracle("cc")), "CC"
     note(f"bs {bs}: CC ok ({it} iterations)")
-    p, it, delta = gbo.pagerank(A, s22.deg, ALPHA, 1e-9, 1000, strict=True)
+    # FP32 iterates stop changing somewhere near 1e-8 L1 at this size: run to
+    # 1e-9 or 300 iterations, then bound the error directly
+    p, it, delta = gbo.pagerank(A, s22.deg, ALPHA, 1e-9, 300)
     p64 = s22.oracle("pagerank")
     l1 = float(np.abs(p.astype(np.float64) - p64).sum())
     # residual bound: needs no reference solution
@@ -193,8 +195,11 @@ def test_rmat22_ppr_forward_adjoint(gbo, po, s22):
     S, W = _ppr_inputs(n)
     A = s22.pack(8)
     t = time.time()
-    P, itf, df = gbo.ppr_forward(A, deg, S, a, 1e-8, 1000, strict=True)
-    gS, ga, ita = gbo.ppr_adjoint(A, A, deg, S, P, W, a, 1e-8, 1000, strict=True)
+    # solves run to 1e-8 (relative L1 change) or 400 iterations, whichever
+    # comes first: the FP32 floor of the change is not known in advance, and
+    # the FP64 residual bounds below measure the error itself
+    P, itf, df = gbo.ppr_forward(A, deg, S, a, 1e-8, 400)
+    gS, ga, ita = gbo.ppr_adjoint(A, A, deg, S, P, W, a, 1e-8, 400)
     note(f"PPR K=64: forward {itf} iterations, adjoint {ita} ({time.time() - t:.1f}s incl. copies)")
     lam = (gS / np.float32(1 - a)).astype(np.float32)
     # forward: per-column 1-norm error bound
@@ -260,7 +265,7 @@ def test_rmat22_ppr_forward_adjoint(gbo, po, s22):
         assert abs(gS[i, cols[k]] - fd) <= 1e-5 * abs(fd) + 1e-7 * scale, (i, k, fd)
     # ... and in alpha, for the loss restricted to the two columns (the GPU
     # adjoint of the two-column problem)
-    _, ga2, _ = gbo.ppr_adjoint(A, A, deg, S[:, cols], P[:, cols], W[:, cols], a, 1e-8, 1000, strict=True)
+    _, ga2, _ = gbo.ppr_adjoint(A, A, deg, S[:, cols], P[:, cols], W[:, cols], a, 1e-8, 400)
     Lp = po.ppr_solve(n, rp, ci, deg, a + h, 0, S2, p64, tol=1e-13, max_iter=1000)[0]
     Lm = po.ppr_solve(n, rp, ci, deg, a - h, 0, S2, p64, tol=1e-13, max_iter=1000)[0]
     fd_a = float((W2 * (Lp - Lm)).sum()) / (2 * h)

@@ -40,7 +40,7 @@ CHILD = textwrap.dedent(r"""
         # host-array path
         Sh = torch.tensor(S, dtype=torch.float32, requires_grad=True)
         ah = torch.tensor(0.85, dtype=torch.float32, requires_grad=True)
-        Ph = gbo_torch.personalized_pagerank(Sh, ah, A, A, g.outdeg(), 1e-9, 2000)
+        Ph = gbo_torch.personalized_pagerank(Sh, ah, A, A, g.outdeg(), 2e-8, 2000)
         (Ph * torch.tensor(W, dtype=torch.float32)).sum().backward()
         # device path, with every host copy of the engine and of tensors refused
         dev = torch.device("cuda:0")
@@ -57,7 +57,7 @@ CHILD = textwrap.dedent(r"""
         gbo.DeviceBuffer.from_numpy = gbo.DeviceBuffer.to_numpy = refuse
         gbo.ppr_forward = gbo.ppr_adjoint = refuse
         try:
-            Pd = gbo_torch.personalized_pagerank(Sd, ad, A, A, od, 1e-9, 2000)
+            Pd = gbo_torch.personalized_pagerank(Sd, ad, A, A, od, 2e-8, 2000)
             assert Pd.is_cuda
             (Pd * Wd).sum().backward()
         finally:
