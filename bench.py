@@ -157,49 +157,70 @@ def time_value_sweeps(gbo, A, g, w_sssp, steps, warmup, timer):
     return out
 
 
-def copy_ceiling(gbo, timer, nbytes=1 << 30, reps=10):
-    """HBM copy ceiling (read + write bytes / time) of a plain copy kernel."""
+def hbm_ceilings(gbo, timer, nbytes=1 << 30, reps=10):
+    """Measured HBM ceilings on this GPU: a plain copy kernel ((read + write)
+    bytes / time) and a read-only stream (read bytes / time), 1 GiB each."""
     L = gbo.lib()
     a = gbo.DeviceBuffer(nbytes)
     b = gbo.DeviceBuffer(nbytes)
-    gbo._check(L.gbo_memset(a.ptr, 1, nbytes))
-    gbo._check(L.gbo_stream_copy(b.ptr, a.ptr, nbytes, None))
-    gbo.synchronize()
-    timer.start()
-    for _ in range(reps):
-        gbo._check(L.gbo_stream_copy(b.ptr, a.ptr, nbytes, None))
-    ms = timer.stop_ms() / reps
-    a.free()
-    b.free()
-    return round(2 * nbytes / (ms * 1e-3) / 1e9, 1)
+    sink = gbo.DeviceBuffer(16)
+    gbo._check(L.gbo_memset(a.ptr, 1, nbytes))  # equal bytes: the XOR fold is 0, never 0xFFFFFFFF
+    out = {}
+    for name, fn, moved in (("copy", lambda: L.gbo_stream_copy(b.ptr, a.ptr, nbytes, None), 2 * nbytes),
+                            ("read", lambda: L.gbo_stream_read(a.ptr, nbytes, 0xFFFFFFFF, sink.ptr, None), nbytes)):
+        gbo._check(fn())
+        gbo.synchronize()
+        timer.start()
+        for _ in range(reps):
+            gbo._check(fn())
+        ms = timer.stop_ms() / reps
+        out[name] = round(moved / (ms * 1e-3) / 1e9, 1)
+    for buf in (a, b, sink):
+        buf.free()
+    return out
 
 
-def bench_multisource(gbo, A, g, nsrc=256, nplain=32, seed=3):
-    """C4a: 256-source batched BFS vs plain per-source BFS launches (same GPU).
-    Throughput = sources * nnz / time.  The plain rate is measured on the
-    first `nplain` sources and scaled per source."""
+def bench_multisource(gbo, A, g, timer, nsrc=256, seed=3):
+    """C4a: 256-source batched BFS vs 256 plain BFS runs on the same GPU, all
+    results in device memory (the plain runs write their levels into their
+    slice of a second [nsrc][n] buffer); the comparison runs after both
+    timings.  Plain runs: pull BFS (gbo_bfs) and direction-optimizing BFS
+    (gbo_bfs_do), each over all 256 sources.  Throughput = sources * nnz /
+    time.  Roofline of the batched pass: per level the tiles, colind and
+    rowptr once plus the frontier and visited words (2 * K * n / 8 bytes)."""
     rng = np.random.default_rng(seed)
     src = rng.integers(0, g.nrows, nsrc).astype(np.int32)
-    out = gbo.DeviceBuffer(4 * nsrc * g.nrows)
+    n = g.nrows
+    isolated = int((g.outdeg()[src] == 0).sum())
+    out = gbo.DeviceBuffer(4 * nsrc * n)
+    plain = gbo.DeviceBuffer(4 * nsrc * n)
     gbo.bfs_multisource(A, src[:64], out)  # warm-up
     gbo.synchronize()
     t = time.perf_counter()
     nl = gbo.bfs_multisource(A, src, out)
     gbo.synchronize()
     tb = time.perf_counter() - t
-    lv_b = out.to_numpy(np.int32, nsrc * g.nrows).reshape(nsrc, g.nrows)
-    exact = True
-    t = time.perf_counter()
-    for i in range(nplain):
-        lv, _ = gbo.bfs(A, int(src[i]))
-        exact &= bool(np.array_equal(lv, lv_b[i]))
-    tp = (time.perf_counter() - t) / nplain
+    res = {"sources": nsrc, "isolated_sources": isolated, "levels": nl, "batched_ms": round(tb * 1e3, 2), "bs": A.bs}
+    for name, kw in (("pull", {}), ("direction_optimizing", {"A": A})):
+        gbo.bfs(A, int(src[0]), out=plain.view(0, 4 * n), **kw)  # warm-up
+        gbo.synchronize()
+        t = time.perf_counter()
+        for i in range(nsrc):
+            gbo.bfs(A, int(src[i]), out=plain.view(4 * i * n, 4 * n), **kw)
+        gbo.synchronize()
+        tp = time.perf_counter() - t
+        exact = bool(np.array_equal(out.to_numpy(np.int32, nsrc * n), plain.to_numpy(np.int32, nsrc * n)))
+        res[f"plain_{name}_ms_per_source"] = round(tp * 1e3 / nsrc, 3)
+        res[f"plain_{name}_GTEPS"] = round(nsrc * g.nnz / tp / 1e9, 3)
+        res[f"speedup_vs_{name}"] = round(tp / tb, 2)
+        res[f"bit_exact_vs_{name}"] = exact
     out.free()
-    rate_b = nsrc * g.nnz / tb / 1e9
-    rate_p = g.nnz / tp / 1e9
-    return {"sources": nsrc, "levels": nl, "batched_ms": round(tb * 1e3, 2), "plain_ms_per_source": round(tp * 1e3, 3),
-            "batched_GTEPS": round(rate_b, 2), "plain_GTEPS": round(rate_p, 3), "speedup": round(rate_b / rate_p, 2),
-            "bit_exact_vs_plain": exact, "plain_sample": nplain, "bs": A.bs}
+    plain.free()
+    res["batched_GTEPS"] = round(nsrc * g.nnz / tb / 1e9, 2)
+    level_bytes = A.tile_bytes() + 4 * A.nblocks + 4 * (A.nblockrows + 1) + 2 * nsrc * n // 8
+    res["bytes_per_level"] = int(level_bytes)
+    res["frac_hbm"] = round(nl * level_bytes / tb / 1e9 / HBM_PEAK_GBS, 4)
+    return res
 
 
 def bench_graph_batch(gbo, count=512, bs=None, seed=4):
@@ -240,17 +261,26 @@ def bench_graph_batch(gbo, count=512, bs=None, seed=4):
         B.run(src, use_graph=True)
     tb = (time.perf_counter() - t) / reps
     levels, labels = B.results()
-    exact = True
-    tp = 0.0
-    for i, g in enumerate(graphs):
-        A = gbo.BsrMatrix.from_csr(g, bs)
-        gbo.synchronize()
-        t = time.perf_counter()
-        lv, _ = gbo.bfs(A, int(src[i]))
-        lb, _ = gbo.cc(A)
-        tp += time.perf_counter() - t
-        exact &= bool(np.array_equal(lv, levels[i]) and np.array_equal(lb, labels[i]))
+    # per-graph plain runs: results stay in device memory (each graph's slice
+    # of two concatenated buffers), compared after the timing
+    off = np.concatenate([[0], np.cumsum([gg.nrows for gg in graphs])])
+    d_lv = gbo.DeviceBuffer(4 * int(off[-1]))
+    d_lb = gbo.DeviceBuffer(4 * int(off[-1]))
+    mats = [gbo.BsrMatrix.from_csr(gg, bs) for gg in graphs]
+    gbo.synchronize()
+    t = time.perf_counter()
+    for i, (gg, A) in enumerate(zip(graphs, mats)):
+        gbo.bfs(A, int(src[i]), out=d_lv.view(4 * int(off[i]), 4 * gg.nrows))
+        gbo.cc(A, out=d_lb.view(4 * int(off[i]), 4 * gg.nrows))
+    gbo.synchronize()
+    tp = time.perf_counter() - t
+    plv, plb = d_lv.to_numpy(np.int32), d_lb.to_numpy(np.int32)
+    exact = all(np.array_equal(plv[off[i]:off[i + 1]], levels[i]) and np.array_equal(plb[off[i]:off[i + 1]], labels[i])
+                for i in range(len(graphs)))
+    for A in mats:
         A.free()
+    d_lv.free()
+    d_lb.free()
     B.free()
     return {"graphs": count, "vertices": int(sum(g.nrows for g in graphs)), "nnz": int(nnz), "bs": bs,
             "autotuned": tuned,
@@ -258,25 +288,42 @@ def bench_graph_batch(gbo, count=512, bs=None, seed=4):
             "plain_ms": round(tp * 1e3, 1), "speedup": round(tp / tb, 2), "bit_exact_vs_plain": exact}
 
 
-def bench_ppr(gbo, A, g, K=64, nseeds=1000, seed=5, tol=1e-6, max_iter=200):
+def bench_ppr(gbo, A, g, timer, K=64, nseeds=1000, seed=5, tol=1e-6, max_iter=200):
     """C5: PPR forward and adjoint, 64 Dirichlet(0.1) personalization
-    vectors over 1000 seed vertices each; w ~ N(0, 1)."""
+    vectors over 1000 seed vertices each; w ~ N(0, 1).  Inputs and outputs
+    are device-resident ([n][K] float32); each solve is timed with HIP
+    events around the C call on the null stream.  Roofline of one SpMM
+    iteration (compulsory bytes): tiles + colind + rowptr + X read once +
+    Y written once + S read once (3 * n * K * 4); the row gathers actually
+    read nnz * K * 4 bytes, reported as gathered_GBs."""
     rng = np.random.default_rng(seed)
-    S = np.zeros((g.nrows, K), np.float32)
+    n = g.nrows
+    S = np.zeros((n, K), np.float32)
     for k in range(K):
-        idx = rng.choice(g.nrows, nseeds, replace=False)
+        idx = rng.choice(n, nseeds, replace=False)
         S[idx, k] = rng.dirichlet(np.full(nseeds, 0.1)).astype(np.float32)
-    W = rng.standard_normal((g.nrows, K)).astype(np.float32)
-    deg = g.outdeg()
-    t = time.perf_counter()
-    P, itf, df = gbo.ppr_forward(A, deg, S, 0.85, tol, max_iter)
-    tf = time.perf_counter() - t
-    t = time.perf_counter()
-    gS, ga, ita = gbo.ppr_adjoint(A, A, deg, S, P, W, 0.85, tol, max_iter)
-    ta = time.perf_counter() - t
-    return {"K": K, "bs": A.bs, "forward_ms": round(tf * 1e3, 1), "forward_iters": itf, "forward_delta": df,
-            "adjoint_ms": round(ta * 1e3, 1), "adjoint_iters": ita, "adjoint_over_forward": round(ta / tf, 3),
-            "ms_per_spmm_iter": round(tf * 1e3 / max(itf, 1), 2), "grad_alpha": ga}
+    W = rng.standard_normal((n, K)).astype(np.float32)
+    d_s, d_w = gbo.DeviceBuffer.from_numpy(S), gbo.DeviceBuffer.from_numpy(W)
+    d_p, d_g = gbo.DeviceBuffer(S.nbytes), gbo.DeviceBuffer(S.nbytes)
+    d_od = gbo.DeviceBuffer.from_numpy(g.outdeg().astype(np.int32))
+    gbo.ppr_forward_ptr(A, d_od.ptr, K, 0.85, d_s.ptr, d_p.ptr, 1e-2, 5, strict=False)  # warm-up
+    gbo.synchronize()
+    timer.start()
+    itf, df = gbo.ppr_forward_ptr(A, d_od.ptr, K, 0.85, d_s.ptr, d_p.ptr, tol, max_iter, strict=False)
+    tf = timer.stop_ms()
+    timer.start()
+    ga, ita = gbo.ppr_adjoint_ptr(A, A, d_od.ptr, K, 0.85, d_s.ptr, d_p.ptr, d_w.ptr, d_g.ptr, tol, max_iter,
+                                  strict=False)
+    ta = timer.stop_ms()
+    for b in (d_s, d_w, d_p, d_g, d_od):
+        b.free()
+    it_bytes = A.tile_bytes() + 4 * A.nblocks + 4 * (A.nblockrows + 1) + 3 * n * K * 4
+    ms_it = tf / max(itf, 1)
+    return {"K": K, "bs": A.bs, "tol": tol, "forward_ms": round(tf, 1), "forward_iters": itf, "forward_delta": df,
+            "adjoint_ms": round(ta, 1), "adjoint_iters": ita, "adjoint_over_forward": round(ta / tf, 3),
+            "ms_per_spmm_iter": round(ms_it, 3), "bytes_per_iter": int(it_bytes),
+            "frac_hbm_per_iter": round(it_bytes / (ms_it * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "gathered_GBs": round(g.nnz * K * 4 / (ms_it * 1e-3) / 1e9, 1), "grad_alpha": ga}
 
 
 def cpu_baseline(gbo, A, xbits, n, target_s):
@@ -323,11 +370,38 @@ def cpu_baseline(gbo, A, xbits, n, target_s):
     }
 
 
+def time_spgemm(gbo, Lr, LTr, timer):
+    """The general SpGEMM C = L * L^T (symbolic + numeric, bit offsets,
+    integer values at C's set bits), timed with HIP events on the null
+    stream around the C calls; returns (ms, C tiles, C nnz, sum of values)."""
+    import ctypes
+    L = gbo.lib()
+    C = gbo.BsrStruct()
+    timer.start()
+    gbo._check(L.gbo_bsrbmm(Lr.ref, LTr.ref, ctypes.byref(C), None))
+    off = gbo.DeviceBuffer(8 * (C.nblocks + 1))
+    nnz = ctypes.c_int64()
+    gbo._check(L.gbo_bsr_bit_offsets(ctypes.byref(C), off.ptr, ctypes.byref(nnz), None))
+    cnt = gbo.DeviceBuffer(4 * max(nnz.value, 1))
+    gbo._check(L.gbo_bsrbmm_values(Lr.ref, LTr.ref, ctypes.byref(C), off.ptr, cnt.ptr, None))
+    ms = timer.stop_ms()
+    total = int(cnt.to_numpy(np.int32, nnz.value).astype(np.int64).sum())
+    nb = C.nblocks
+    L.gbo_bsr_free(ctypes.byref(C))
+    off.free()
+    cnt.free()
+    return ms, nb, nnz.value, total
+
+
 def bench_tc(gbo, timer, reps=3):
     """C3: triangle counting by bsrbmm (L row tiles x L^T column tiles,
     masked by L, popcount, uint64 total) on the 2048x2048 king grid (closed
-    form 4(m-1)^2) and R-MAT s20 (seed 2).  The kernel is timed with HIP
-    events on prepacked operands; strict tril/transposition run on the host."""
+    form 4(m-1)^2) and R-MAT s20 (seed 2).  The whole gbo_bsrbmm_tc call
+    (block transpose of L^T, row form of its tiles, the masked product) is
+    timed with HIP events on prepacked operands; strict tril/transposition
+    run on the host.  bytes = both operands' tiles + colind + rowptr read
+    once.  On the king grid the general SpGEMM C = L * L^T (sum of C's values
+    = sum_c colcount_c(L)^2) is timed as well."""
     out = {}
     for name, g in (("king2048", gbo.gen_king(2048)), ("rmat20", gbo.gen_rmat(20, 16, seed=2))):
         L = g.tril(strict=True)
@@ -344,10 +418,20 @@ def bench_tc(gbo, timer, reps=3):
             for _ in range(reps):
                 cnt = gbo.bsrbmm_tc(Lr, Lc)
             ms = timer.stop_ms() / reps
+            byt = 2 * (Lr.tile_bytes() + 4 * Lr.nblocks + 4 * (Lr.nblockrows + 1))
             per[bs] = {"triangles": cnt, "ms": round(ms, 3), "L_tiles": Lr.nblocks,
-                       "GTEPS_L": round(L.nnz / (ms * 1e-3) / 1e9, 3)}
-            Lr.free()
+                       "GTEPS_L": round(L.nnz / (ms * 1e-3) / 1e9, 3), "bytes": int(byt),
+                       "frac_hbm": round(byt / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
             Lc.free()
+            if name == "king2048":
+                LTr = gbo.BsrMatrix.from_csr(LT, bs, 0)
+                time_spgemm(gbo, Lr, LTr, timer)  # warm-up
+                sms, cnb, cnnz, csum = time_spgemm(gbo, Lr, LTr, timer)
+                want = int((np.bincount(L.colind, minlength=g.ncols).astype(np.int64) ** 2).sum())
+                per[bs]["spgemm_LLT"] = {"ms": round(sms, 3), "C_tiles": cnb, "C_nnz": cnnz, "values_sum": csum,
+                                         "equals_colcount_squares": csum == want}
+                LTr.free()
+            Lr.free()
         rec["per_block_size"] = per
         counts = {v["triangles"] for v in per.values()}
         rec["consistent_across_bs"] = len(counts) == 1
@@ -356,6 +440,35 @@ def bench_tc(gbo, timer, reps=3):
         out[name] = rec
         log(f"[bench] tc {name}: {rec}")
     return out
+
+
+def bench_ingest(gbo, g, workdir):
+    """.mtx ingest at scale: the graph written as a pattern .mtx file, then
+    read with both reader semantics, then through the binary CSR cache
+    (first call parses and writes it, second loads it).  Host work only."""
+    import tempfile
+    rec = {"threads": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))}
+    with tempfile.TemporaryDirectory(dir=workdir) as d:
+        p = os.path.join(d, "graph.mtx")
+        t = time.perf_counter()
+        gbo.mtx_write(p, g)
+        rec["write_s"] = round(time.perf_counter() - t, 2)
+        rec["mtx_GB"] = round(os.path.getsize(p) / 1e9, 3)
+        for sem in (0, 1):
+            t = time.perf_counter()
+            c = gbo.mtx_read(p, sem, 0)
+            rec[f"read_semantics{sem}_s"] = round(time.perf_counter() - t, 2)
+            rec[f"semantics{sem}_equal"] = bool(np.array_equal(c.rowptr, g.rowptr) and
+                                                np.array_equal(c.colind, g.colind))
+            del c
+        for key in ("cache_write_s", "cache_load_s"):
+            t = time.perf_counter()
+            c = gbo.mtx_read(p, 1, 0, cache=True)
+            rec[key] = round(time.perf_counter() - t, 2)
+            rec[key.replace("_s", "_state")] = c.cache_state
+        rec["cache_equal"] = bool(np.array_equal(c.rowptr, g.rowptr) and np.array_equal(c.colind, g.colind))
+        rec["cache_GB"] = round(os.path.getsize(gbo.mtx_cache_path(p, 1, 0)) / 1e9, 3)
+    return rec
 
 
 def keep_ppr(gbo, d_rp, d_ci, n):
@@ -375,6 +488,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--variants", default="0,1,2,3", help="bs 8/16 kernel variants to time (best is kept)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched BFS / graph batch / PPR lines")
+    ap.add_argument("--ingest", action="store_true",
+                    help="only time .mtx ingest of the graph (write, both reader semantics, binary cache)")
+    ap.add_argument("--ingest-dir", default=None, help="directory for the --ingest files (default: system temp)")
     ap.add_argument("--profile", action="store_true",
                     help="short run for rocprofv3: only the count sweeps and the copy kernel")
     ap.add_argument("--profile-sweeps", action="store_true",
@@ -393,12 +509,16 @@ def main():
     g = gbo.gen_rmat(args.scale, 16, 0.57, 0.19, 0.19, seed=1)
     n, nnz = g.nrows, g.nnz
     log(f"[bench] R-MAT s{args.scale}: n={n} nnz={nnz} generated in {time.time() - t:.1f}s")
+    if args.ingest:
+        print(json.dumps({"ingest": {"graph": f"R-MAT s{args.scale}", "n": n, "nnz": nnz,
+                                     **bench_ingest(gbo, g, args.ingest_dir)}}), flush=True)
+        return
     d_rp = gbo.DeviceBuffer.from_numpy(g.rowptr)
     d_ci = gbo.DeviceBuffer.from_numpy(g.colind)
     outdeg = g.outdeg()
     timer = Timer(gbo)
-    copy_gbs = copy_ceiling(gbo, timer)
-    log(f"[bench] copy-kernel ceiling {copy_gbs} GB/s")
+    ceil = hbm_ceilings(gbo, timer)
+    log(f"[bench] HBM ceilings: copy {ceil['copy']} GB/s, read-only {ceil['read']} GB/s")
     if args.profile_sweeps:
         # BFS full steps and value sweeps only, each block size's default plan
         w = gbo.edge_weights(g, 1, 64, 1)
@@ -533,9 +653,9 @@ def main():
     A, dx = keep[bs]
     extra = {}
     if not args.no_batch:
-        for name, fn in (("multisource_bfs", lambda: bench_multisource(gbo, A, g)),
+        for name, fn in (("multisource_bfs", lambda: bench_multisource(gbo, A, g, timer)),
                          ("graph_batch", lambda: bench_graph_batch(gbo)),
-                         ("ppr", lambda: bench_ppr(gbo, keep_ppr(gbo, d_rp, d_ci, n), g))):
+                         ("ppr", lambda: bench_ppr(gbo, keep_ppr(gbo, d_rp, d_ci, n), g, timer))):
             try:
                 t = time.time()
                 extra[name] = fn()
@@ -564,7 +684,9 @@ def main():
                    "nnz": nnz, "bs": bs, "parallelism": "none (1 GPU)"},
         "roofline": {"bound": "hbm", "achieved": best["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": best["frac_hbm"], "traffic": measured_traffic(bs, best["variant"], args.scale),
-                     "algorithmic_bytes": best["bytes_per_sweep"], "copy_ceiling_GBs": copy_gbs},
+                     "algorithmic_bytes": best["bytes_per_sweep"], "copy_ceiling_GBs": ceil["copy"],
+                     "read_ceiling_GBs": ceil["read"],
+                     "frac_of_read_ceiling": round(best["achieved_GBs"] / ceil["read"], 4)},
         "cpu_baseline": cpu,
         "per_block_size": per_bs,
         "triangle_counting": tc,

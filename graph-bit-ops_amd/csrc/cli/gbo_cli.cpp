@@ -185,9 +185,11 @@ int run_bsrbmv(const Opts& o) {
   return ok ? 0 : 1;
 }
 
-// bsrbmm<N> (sum of L L^T, printed as ntris_bmm like the reference) and
-// tc<N> (masked: triangle count).  L = tril(A) keeps row >= col
-// (tril_csr, bsrbmm/bsrbmm.cu:1416).
+// bsrbmm<N> (the product C = L L^T by the bit-BSR SpGEMM -- symbolic,
+// numeric and the integer values at C's set bits, the timed work -- and the
+// sum of C's values printed as ntris_bmm like the reference) and tc<N>
+// (masked: triangle count).  L = tril(A) keeps row >= col (tril_csr,
+// bsrbmm/bsrbmm.cu:1416).
 int run_bmm(const Opts& o, bool masked) {
   HostCsr c;
   read_mtx(o, masked ? 1 : 0, &c);
@@ -197,22 +199,44 @@ int run_bmm(const Opts& o, bool masked) {
   CK(gbo_csr_tril(c.nrows, c.rp, c.ci, 0, &lnnz, &lp, &li));
   std::printf("nnz before tril_csr(): %lld, after tril_csr(): %lld\n", (long long)c.nnz, (long long)lnnz);
   CK(gbo_csr_transpose(c.nrows, c.ncols, lp, li, &tp, &ti));
-  gbo_bsr L{}, LT{};
+  gbo_bsr L{}, LT{}, C{};
   pack(o, 0, c.nrows, c.ncols, lp, li, &L);
-  CK(gbo_bsr_create(o.bs, 1, tp, ti, c.ncols, c.nrows, &LT, nullptr));
+  // tc: L^T column-packed (the masked kernel's operand); bsrbmm: L^T
+  // row-packed, the SpGEMM's right operand
+  CK(gbo_bsr_create(o.bs, masked ? 1 : 0, tp, ti, c.ncols, c.nrows, &LT, nullptr));
   uint64_t* d = dalloc<uint64_t>(1);
+  int64_t* d_off = nullptr;
+  int32_t* d_cnt = nullptr;
+  int64_t cnnz = 0;
   void *e0, *e1;
   CK(gbo_event_create(&e0));
   CK(gbo_event_create(&e1));
   CK(gbo_synchronize());
   CK(gbo_event_record(e0, nullptr));
-  if (masked) CK(gbo_bsrbmm_tc(&L, &LT, d, nullptr));
-  else CK(gbo_bsrbmm_sum(&L, &LT, d, nullptr));
+  if (masked) {
+    CK(gbo_bsrbmm_tc(&L, &LT, d, nullptr));
+  } else {
+    CK(gbo_bsrbmm(&L, &LT, &C, nullptr));
+    d_off = dalloc<int64_t>((size_t)C.nblocks + 1);
+    CK(gbo_bsr_bit_offsets(&C, d_off, &cnnz, nullptr));
+    d_cnt = dalloc<int32_t>((size_t)cnnz);
+    CK(gbo_bsrbmm_values(&L, &LT, &C, d_off, d_cnt, nullptr));
+  }
   CK(gbo_event_record(e1, nullptr));
   float ms = 0;
   CK(gbo_event_elapsed(&ms, e0, e1));
   uint64_t r = 0;
-  CK(gbo_memcpy_d2h(&r, d, 8));
+  if (masked) {
+    CK(gbo_memcpy_d2h(&r, d, 8));
+  } else {
+    std::vector<int32_t> v((size_t)cnnz);
+    if (cnnz) CK(gbo_memcpy_d2h(v.data(), d_cnt, sizeof(int32_t) * (size_t)cnnz));
+    for (int32_t x : v) r += (uint64_t)x;
+    std::printf("C = L*L^T: nblocks %lld, nnz %lld\n", (long long)C.nblocks, (long long)cnnz);
+    gbo_free(d_off);
+    gbo_free(d_cnt);
+    gbo_bsr_free(&C);
+  }
   std::printf("ntris_bmm: %llu\n", (unsigned long long)r);
   if (!masked) {  // host check: sum_c colcount_c(L)^2 (the cusparseScsrgemm sum)
     std::vector<int64_t> cc(c.ncols, 0);

@@ -81,7 +81,7 @@ class AsyncAllocs {
   }
 
  private:
-  static constexpr int kMax = 8;
+  static constexpr int kMax = 16;
   hipStream_t st_;
   void* ptrs_[kMax] = {};
   int n_ = 0;

@@ -7,10 +7,46 @@
 namespace {
 
 // Grid-stride 16-byte copy: the plain copy used to measure the HBM copy
-// ceiling and to calibrate the FETCH_SIZE / WRITE_SIZE counters.
-__global__ __launch_bounds__(256) void k_stream_copy(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n16) {
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (size_t)gridDim.x * blockDim.x)
-    dst[i] = src[i];
+// ceiling and to calibrate the FETCH_SIZE / WRITE_SIZE counters.  Each
+// thread issues four loads before their stores (64 bytes in flight per
+// lane); the stores are nontemporal.
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(256) void k_stream_copy(const u32x4* __restrict__ src, u32x4* __restrict__ dst, size_t n16) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (; i + 3 * stride < n16; i += 4 * stride) {
+    u32x4 v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) v[u] = __builtin_nontemporal_load(src + i + u * stride);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) __builtin_nontemporal_store(v[u], dst + i + u * stride);
+  }
+  for (; i < n16; i += stride) dst[i] = src[i];
+}
+
+// Read-only stream: every 16-byte word is loaded once (four in flight per
+// lane) and folded into one XOR per thread, stored only when it equals
+// `never` (a value the caller knows the fold cannot take), so the loads
+// cannot be dropped and nothing is written in practice -- the HBM read
+// ceiling.
+__global__ __launch_bounds__(256) void k_stream_read(const u32x4* __restrict__ src, size_t n16, uint32_t never,
+                                                     uint32_t* __restrict__ sink) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t x = 0;
+  for (; i + 3 * stride < n16; i += 4 * stride) {
+    u32x4 v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) v[u] = __builtin_nontemporal_load(src + i + u * stride);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) x ^= v[u].x ^ v[u].y ^ v[u].z ^ v[u].w;
+  }
+  for (; i < n16; i += stride) {
+    const u32x4 v = src[i];
+    x ^= v.x ^ v.y ^ v.z ^ v.w;
+  }
+  if (x == never) sink[0] = x;
 }
 
 }  // namespace
@@ -23,7 +59,18 @@ int gbo_stream_copy(void* d_dst, const void* d_src, size_t bytes, gbo_stream_t s
   const size_t n16 = bytes / 16;
   if (n16 == 0) return GBO_OK;
   const size_t blocks = std::min<size_t>((n16 + 255) / 256, 256 * 64);
-  k_stream_copy<<<(unsigned)blocks, 256, 0, stream>>>((const uint4*)d_src, (uint4*)d_dst, n16);
+  k_stream_copy<<<(unsigned)blocks, 256, 0, stream>>>((const u32x4*)d_src, (u32x4*)d_dst, n16);
+  GBO_LAUNCH_CHECK();
+  return GBO_OK;
+}
+
+int gbo_stream_read(const void* d_src, size_t bytes, uint32_t never, uint32_t* d_sink, gbo_stream_t stream) {
+  if (bytes % 16 || (uintptr_t)d_src % 16 || !d_sink)
+    return gbo::set_error(GBO_EINVAL, "gbo_stream_read: size and pointer must be 16-byte aligned, sink non-null");
+  const size_t n16 = bytes / 16;
+  if (n16 == 0) return GBO_OK;
+  const size_t blocks = std::min<size_t>((n16 + 255) / 256, 256 * 64);
+  k_stream_read<<<(unsigned)blocks, 256, 0, stream>>>((const u32x4*)d_src, n16, never, d_sink);
   GBO_LAUNCH_CHECK();
   return GBO_OK;
 }

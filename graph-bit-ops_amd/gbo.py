@@ -96,6 +96,7 @@ def _declare(L):
         "gbo_event_elapsed": ([POINTER(c_float), P, P], c_int),
         "gbo_free_host": ([P], None),
         "gbo_stream_copy": ([P, P, c_size_t, P], c_int),
+        "gbo_stream_read": ([P, c_size_t, c_uint32, P, P], c_int),
         "gbo_mtx_read": ([c_char_p, c_int, c_int, i32p, i32p, i64p, POINTER(P), POINTER(P), POINTER(P)], c_int),
         "gbo_mtx_read_cached": ([c_char_p, c_int, c_int, POINTER(c_int), i32p, i32p, i64p, POINTER(P), POINTER(P),
                                  POINTER(P)], c_int),
@@ -127,6 +128,11 @@ def _declare(L):
         "gbo_bsrbmv_min_i32": ([POINTER(BsrStruct), P, P, P], c_int),
         "gbo_bsrbmm_tc": ([POINTER(BsrStruct), POINTER(BsrStruct), P, P], c_int),
         "gbo_bsrbmm_sum": ([POINTER(BsrStruct), POINTER(BsrStruct), P, P], c_int),
+        "gbo_bsrbmm_symbolic": ([POINTER(BsrStruct), POINTER(BsrStruct), P, i64p, P], c_int),
+        "gbo_bsrbmm_numeric": ([POINTER(BsrStruct), POINTER(BsrStruct), P, c_int64, P, P, P, i64p, P], c_int),
+        "gbo_bsrbmm_values": ([POINTER(BsrStruct), POINTER(BsrStruct), POINTER(BsrStruct), P, P, P], c_int),
+        "gbo_bsrbmm": ([POINTER(BsrStruct), POINTER(BsrStruct), POINTER(BsrStruct), P], c_int),
+        "gbo_bsr_bit_offsets": ([POINTER(BsrStruct), P, i64p, P], c_int),
         "gbo_bfs": ([POINTER(BsrStruct), c_int32, P, P, c_int32, i32p, P], c_int),
         "gbo_bfs_do": ([POINTER(BsrStruct), POINTER(BsrStruct), c_int32, P, P, c_int32, i32p, P], c_int),
         "gbo_sssp": ([POINTER(BsrStruct), P, P, c_int32, P, i32p, P], c_int),
@@ -247,6 +253,12 @@ class DeviceBuffer:
         if arr.nbytes:
             _check(lib().gbo_memcpy_h2d(self.ptr, arr.ctypes.data, arr.nbytes))
 
+    def view(self, offset: int, nbytes: int) -> "DeviceView":
+        """A non-owning window [offset, offset + nbytes) of this allocation."""
+        if offset < 0 or offset + nbytes > self.nbytes:
+            raise ValueError(f"view [{offset}, {offset + nbytes}) outside {self.nbytes} bytes")
+        return DeviceView(self.ptr + offset, nbytes)
+
     def free(self):
         if getattr(self, "ptr", None):
             lib().gbo_free(self.ptr)
@@ -260,6 +272,20 @@ class DeviceBuffer:
 
 
 # ----------------------------------------------------------------- CSR host
+class DeviceView(DeviceBuffer):
+    """A window of a DeviceBuffer (freeing it frees nothing)."""
+
+    def __init__(self, ptr: int, nbytes: int):  # noqa: D107 -- no allocation
+        self.ptr = int(ptr)
+        self.nbytes = int(nbytes)
+
+    def free(self):
+        self.ptr = None
+
+    def __del__(self):
+        pass
+
+
 class Csr:
     """Host CSR (int32 rowptr/colind), as produced by the readers/generators."""
 
@@ -441,6 +467,51 @@ class BsrMatrix:
             self.free()
         except Exception:
             pass
+
+
+def bsrbmm(A: BsrMatrix, B: BsrMatrix, counts: bool = False):
+    """General bit-BSR SpGEMM C = A * B (both row-packed): the Boolean product
+    as a library-owned BsrMatrix; with counts=True also (valoff, counts) as
+    host arrays -- the integer product at C's set bits in tile bit order."""
+    s = BsrStruct()
+    _check(lib().gbo_bsrbmm(A.ref, B.ref, byref(s), None))
+    C = BsrMatrix(s)
+    if not counts:
+        return C
+    nb = C.nblocks
+    d_off = DeviceBuffer(8 * (nb + 1))
+    nnz = c_int64()
+    _check(lib().gbo_bsr_bit_offsets(C.ref, d_off.ptr, byref(nnz), None))
+    d_cnt = DeviceBuffer(4 * max(nnz.value, 1))
+    _check(lib().gbo_bsrbmm_values(A.ref, B.ref, C.ref, d_off.ptr, d_cnt.ptr, None))
+    off = d_off.to_numpy(np.int64, nb + 1)
+    cnt = d_cnt.to_numpy(np.int32, nnz.value)
+    d_off.free()
+    d_cnt.free()
+    return C, off, cnt
+
+
+def bsrbmm_phases(A: BsrMatrix, B: BsrMatrix):
+    """The two-phase, caller-allocates form (symbolic sizes C, numeric fills
+    it), as the reference's bmm32_sparse_symbolic / _numeric pair is called.
+    Returns host (rowptr, colind, tiles, valoff, nnz)."""
+    L = lib()
+    nbr = A.nblockrows
+    d_rp = DeviceBuffer(4 * (nbr + 1))
+    nb = c_int64()
+    _check(L.gbo_bsrbmm_symbolic(A.ref, B.ref, d_rp.ptr, byref(nb), None))
+    n = nb.value
+    bs = A.bs
+    d_ci = DeviceBuffer(4 * max(n, 1))
+    d_t = DeviceBuffer(max(n * bs * bs // 8, 8))
+    d_off = DeviceBuffer(8 * (n + 1))
+    nnz = c_int64()
+    _check(L.gbo_bsrbmm_numeric(A.ref, B.ref, d_rp.ptr, n, d_ci.ptr, d_t.ptr, d_off.ptr, byref(nnz), None))
+    out = (d_rp.to_numpy(np.int32, nbr + 1), d_ci.to_numpy(np.int32, n),
+           d_t.to_numpy(WORD_DTYPE[bs], n * bs), d_off.to_numpy(np.int64, n + 1), nnz.value)
+    for b in (d_rp, d_ci, d_t, d_off):
+        b.free()
+    return out
 
 
 def csr2bsr_batch(csr: Csr, bs: int, direction: int = 0):

@@ -101,6 +101,12 @@ void gbo_free_host(void* h_ptr);
  * the roofline and calibrates the FETCH_SIZE/WRITE_SIZE counters.  No
  * reference counterpart. */
 int gbo_stream_copy(void* d_dst, const void* d_src, size_t bytes, gbo_stream_t stream);
+/* Read-only stream over d_src (16-byte aligned): every word is loaded once
+ * and XOR-folded per thread; a thread stores its fold to *d_sink only when
+ * it equals `never` (pick a value the fold cannot take for the data, e.g.
+ * 0xFFFFFFFF over a buffer of equal bytes), so nothing is written in
+ * practice.  Measures the HBM read ceiling.  No reference counterpart. */
+int gbo_stream_read(const void* d_src, size_t bytes, uint32_t never, uint32_t* d_sink, gbo_stream_t stream);
 
 /* ------------------------------------------------- Matrix Market readers --
  * semantics 0: mmio_info + mmio_data (algorithm/mmio_highlevel.h:7,146):
@@ -273,6 +279,43 @@ int gbo_bsrbmm_tc(const gbo_bsr* L_row, const gbo_bsr* LT_col, uint64_t* d_count
  * B = L^T and prints it as "ntris_bmm"). */
 int gbo_bsrbmm_sum(const gbo_bsr* A_row, const gbo_bsr* B_col, uint64_t* d_sum,
                    gbo_stream_t stream);
+
+/* ------------------------------------------- general bit-BSR SpGEMM ------
+ * C = A * B with a bit-BSR output: A and B row-packed (dir 0), same block
+ * size, A->ncols == B->nrows; C is row-packed, A->nrows x B->ncols.  A tile
+ * of C exists exactly when the Boolean product has a set bit in it (the
+ * tiles gbo_csr2bsr_batch would make of the product's pattern); C's tile
+ * words are the Boolean product (bit set iff (A B)_ij > 0).  Expand-sort-
+ * compress over the productive tile pairs (colmask(A tile) & rowmask(B
+ * tile) != 0) on the device; each phase allocates and frees its own scratch.
+ *
+ * Symbolic phase (bmm32_sparse_symbolic, bsrbmm/bsrbmm.cu:1273): C's block
+ * row pointer into the caller's d_C_rowptr[A->nblockrows + 1] and the tile
+ * count into *C_nblocks (host). */
+int gbo_bsrbmm_symbolic(const gbo_bsr* A, const gbo_bsr* B, int32_t* d_C_rowptr, int64_t* C_nblocks,
+                        gbo_stream_t stream);
+/* Numeric phase (bmm32_sparse_numeric, bsrbmm/bsrbmm.cu:1311): C's block
+ * columns d_C_colind[C_nblocks] (ascending per block row) and tile words
+ * d_C_tiles[C_nblocks * bs], caller-allocated from the symbolic phase's
+ * count.  With d_C_valoff (int64[C_nblocks + 1], may be NULL) also the
+ * exclusive scan of the tiles' bit counts, and *C_nnz (host, may be NULL)
+ * their total: the offsets gbo_bsrbmm_values writes the counts at. */
+int gbo_bsrbmm_numeric(const gbo_bsr* A, const gbo_bsr* B, const int32_t* d_C_rowptr, int64_t C_nblocks,
+                       int32_t* d_C_colind, void* d_C_tiles, int64_t* d_C_valoff, int64_t* C_nnz,
+                       gbo_stream_t stream);
+/* The integer product (A B)_ij = sum_k A_ik B_kj at every set bit of C (the
+ * values bmm32_sparse_numeric stores in C_csrval), int32, in tile bit order
+ * (tiles in C's order, rows in order, columns ascending inside a row) from
+ * d_C_valoff[t] (gbo_bsrbmm_numeric). */
+int gbo_bsrbmm_values(const gbo_bsr* A, const gbo_bsr* B, const gbo_bsr* C, const int64_t* d_C_valoff,
+                      int32_t* d_C_counts, gbo_stream_t stream);
+/* Both phases into a library-owned C (release with gbo_bsr_free), with the
+ * kernel variant gbo_bsr_create_dev would pick and no plan yet. */
+int gbo_bsrbmm(const gbo_bsr* A, const gbo_bsr* B, gbo_bsr* C, gbo_stream_t stream);
+/* Exclusive scan of the tiles' bit counts, d_off[A->nblocks + 1] (int64),
+ * and their total *nnz (host, may be NULL): the offsets of per-nonzero
+ * values in tile bit order. */
+int gbo_bsr_bit_offsets(const gbo_bsr* A, int64_t* d_off, int64_t* nnz, gbo_stream_t stream);
 
 /* ------------------------------------------------------------- drivers --
  * Whole algorithms with device-resident loops (host reads one scalar per

@@ -329,6 +329,58 @@ uint64_t or_bsrbmm_sum(int bs, int32_t nbr, const int32_t* arp, const int32_t* a
   return sum;
 }
 
+/* C = A * B over the integers, element-level CSR (what bmm32_sparse_symbolic
+ * + bmm32_sparse_numeric, bsrbmm/bsrbmm.cu:1273,1311, set out to produce:
+ * C's row pointer, then per row the columns with a nonzero product, each with
+ * its count sum_k A_ik B_kj as the value).  Gustavson with a dense marker per
+ * thread; columns ascending.  First call with ci == NULL fills rp and returns
+ * nnz; second call fills ci/val. */
+int64_t or_spgemm(int32_t m, const int32_t* arp, const int32_t* aci, const int32_t* brp, const int32_t* bci,
+                  int32_t ncols, int32_t* rp, int32_t* ci, int32_t* val) {
+  if (!ci) {
+#pragma omp parallel
+    {
+      int32_t* mark = (int32_t*)malloc(sizeof(int32_t) * (size_t)(ncols > 0 ? ncols : 1));
+      for (int32_t j = 0; j < ncols; ++j) mark[j] = -1;
+#pragma omp for schedule(dynamic, 256)
+      for (int32_t i = 0; i < m; ++i) {
+        int32_t c = 0;
+        for (int32_t p = arp[i]; p < arp[i + 1]; ++p)
+          for (int32_t q = brp[aci[p]]; q < brp[aci[p] + 1]; ++q)
+            if (mark[bci[q]] != i) {
+              mark[bci[q]] = i;
+              ++c;
+            }
+        rp[i + 1] = c;
+      }
+      free(mark);
+    }
+    rp[0] = 0;
+    for (int32_t i = 0; i < m; ++i) rp[i + 1] += rp[i];
+    return rp[m];
+  }
+#pragma omp parallel
+  {
+    int32_t* acc = (int32_t*)calloc((size_t)(ncols > 0 ? ncols : 1), sizeof(int32_t));
+#pragma omp for schedule(dynamic, 256)
+    for (int32_t i = 0; i < m; ++i) {
+      int32_t c = rp[i];
+      for (int32_t p = arp[i]; p < arp[i + 1]; ++p)
+        for (int32_t q = brp[aci[p]]; q < brp[aci[p] + 1]; ++q) {
+          if (acc[bci[q]] == 0) ci[c++] = bci[q];
+          acc[bci[q]]++;
+        }
+      qsort(ci + rp[i], (size_t)(c - rp[i]), sizeof(int32_t), cmp_i32);
+      for (int32_t k = rp[i]; k < c; ++k) {
+        val[k] = acc[ci[k]];
+        acc[ci[k]] = 0;
+      }
+    }
+    free(acc);
+  }
+  return rp[m];
+}
+
 /* -------------------------------------------------------------- drivers -- */
 int32_t or_bfs_bsr(int bs, int32_t n, const int32_t* brp, const int32_t* bci, const void* tiles, int32_t src,
                    int32_t* level, int64_t* sizes, int32_t max_levels) {

@@ -77,6 +77,10 @@ uint64_t or_bsrbmm_masked(int bs, int32_t nbr, const int32_t* arp, const int32_t
 /* bmm{N}_sparse (bsrbmm/bsrbmm.cu:434-488): sum_{i,k} (A B)_ik, tile-pair loop. */
 uint64_t or_bsrbmm_sum(int bs, int32_t nbr, const int32_t* arp, const int32_t* aci, const void* atiles,
                        const int32_t* brp, const int32_t* bci, const void* btiles);
+/* C = A * B over the integers, element CSR with counts (bsrbmm/bsrbmm.cu:1273,
+ * 1311); ci == NULL: fill rp[m+1], return nnz; else fill ci/val. */
+int64_t or_spgemm(int32_t m, const int32_t* arp, const int32_t* aci, const int32_t* brp, const int32_t* bci,
+                  int32_t ncols, int32_t* rp, int32_t* ci, int32_t* val);
 
 /* ---- drivers restated on bit tiles ---- */
 /* BFS loop of algorithm/bfs.cu:508-564 (visited |= f; f = A f & ~visited)

@@ -47,6 +47,7 @@ def lib():
             "or_bsrbmv_min_i32": ([c_int, c_int32, P, P, P, P, P], None),
             "or_bsrbmm_masked": ([c_int, c_int32, P, P, P, P, P, P], c_uint64),
             "or_bsrbmm_sum": ([c_int, c_int32, P, P, P, P, P, P], c_uint64),
+            "or_spgemm": ([c_int32, P, P, P, P, c_int32, P, P, P], c_int64),
             "or_bfs_bsr": ([c_int, c_int32, P, P, P, c_int32, P, P, c_int32], c_int32),
             "or_bfs_queue": ([c_int32, P, P, c_int32, P], c_int32),
             "or_dijkstra": ([c_int32, P, P, P, c_int32, P], None),
@@ -190,6 +191,17 @@ def bsrbmm_masked(bs, arp, aci, atiles, brp, bci, btiles):
 
 def bsrbmm_sum(bs, arp, aci, atiles, brp, bci, btiles):
     return lib().or_bsrbmm_sum(bs, len(arp) - 1, _p(arp), _p(aci), _p(atiles), _p(brp), _p(bci), _p(btiles))
+
+
+def spgemm(m, arp, aci, brp, bci, ncols):
+    """C = A * B over the integers as element CSR (rowptr, colind, counts)."""
+    arp, aci, brp, bci = (np.ascontiguousarray(a, np.int32) for a in (arp, aci, brp, bci))
+    rp = np.zeros(m + 1, np.int32)
+    nnz = lib().or_spgemm(m, _p(arp), _p(aci), _p(brp), _p(bci), ncols, _p(rp), None, None)
+    ci = np.zeros(max(nnz, 1), np.int32)
+    val = np.zeros(max(nnz, 1), np.int32)
+    lib().or_spgemm(m, _p(arp), _p(aci), _p(brp), _p(bci), ncols, _p(rp), _p(ci), _p(val))
+    return rp, ci[:nnz].copy(), val[:nnz].copy()
 
 
 # ----------------------------------------------------------------- drivers

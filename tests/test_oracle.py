@@ -489,3 +489,38 @@ def test_ppr_check_bounds():
     _, linf, _, _ = po.ppr_check(n, rp, ci, deg, a, 1, Lp, W)
     true = np.abs(Lp.astype(np.float32) - lam).max(0)
     assert np.all(true <= linf / (1 - a) + 1e-5)
+
+
+@pytest.mark.parametrize("case", ["rect", "rmat", "empty", "directed"])
+def test_spgemm_against_scipy(po, case):
+    """or_spgemm (the general product's checker): pattern and integer counts
+    equal scipy's A @ B on rectangular, skewed, empty and directed operands
+    (a transposed operand or a dropped term changes the counts)."""
+    import scipy.sparse as sp
+    if case == "rect":
+        A = sp.random(70, 90, 0.05, format="csr", random_state=1)
+        B = sp.random(90, 40, 0.08, format="csr", random_state=2)
+    elif case == "rmat":
+        import gbo
+        g = gbo.gen_rmat(9, 8, 0.57, 0.19, 0.19, seed=4)
+        A = B = sp.csr_matrix((np.ones(g.nnz), g.colind, g.rowptr), shape=(g.nrows, g.nrows))
+    elif case == "empty":
+        A = sp.csr_matrix((30, 50))
+        B = sp.random(50, 20, 0.2, format="csr", random_state=3)
+    else:
+        A = sp.random(60, 60, 0.06, format="csr", random_state=5)
+        B = A.T.tocsr()
+    A, B = A.tocsr(), B.tocsr()
+    A.data[:] = 1
+    B.data[:] = 1
+    A.sort_indices()
+    B.sort_indices()
+    rp, ci, val = po.spgemm(A.shape[0], A.indptr, A.indices, B.indptr, B.indices, B.shape[1])
+    C = (A @ B).tocsr()
+    C.eliminate_zeros()
+    C.sort_indices()
+    assert np.array_equal(rp, C.indptr) and np.array_equal(ci, C.indices)
+    assert np.array_equal(val, C.data.astype(np.int32))
+    if case == "directed":  # A A^T differs from A^T A: the operands are not swapped
+        rp2, ci2, _ = po.spgemm(60, B.indptr, B.indices, A.indptr, A.indices, 60)
+        assert not (np.array_equal(rp, rp2) and np.array_equal(ci, ci2))
