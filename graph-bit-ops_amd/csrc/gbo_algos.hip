@@ -10,6 +10,7 @@
 // convergence reduction); the host reads one 8-byte scalar per iteration.
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 #include "gbo_device.hpp"
@@ -120,26 +121,48 @@ __global__ __launch_bounds__(256) void k_popc_words(const word_t<BS>* __restrict
   if (lane_id() == 0 && c) atomicAdd(cnt, (unsigned long long)c);
 }
 
+// Scratch of a BFS call supplied by the caller of bfs_t instead of being
+// allocated: frontier / next / visited words (3 x nbr words), the frontier-
+// size slots (kCountSlots, cleared on entry) and count, and (bs 8, variant
+// 3) the dense step's row counts (8 x (nbr + 1) int32, else null).
+struct BfsScratch {
+  void* words;
+  unsigned long long* slots;
+  int64_t* count;
+  int32_t* dense_counts;
+};
+
 // Pull BFS over AT; with A (the out-edge matrix, AT itself for an undirected
 // graph) levels whose frontier holds at most n/64 vertices are pushed.  The
 // frontier-size slots and the dense step's row counts come from the plan's
-// scratch; the frontier, next and visited words are this call's.
+// scratch; the frontier, next and visited words are this call's -- or, with
+// `ext` (pull only), all of them come from there.
 template <int BS>
 static int bfs_t(const gbo_bsr* A, const gbo_bsr* Aout, int32_t source, int32_t* d_level, int64_t* sizes,
-                 int32_t max_levels, int32_t* nlevels, hipStream_t st) {
+                 int32_t max_levels, int32_t* nlevels, hipStream_t st, const BfsScratch* ext = nullptr) {
   using W = word_t<BS>;
   const int32_t nbr = A->nblockrows, n = A->nrows;
   AsyncAllocs mem(st);
   W* f = nullptr;
-  int rc = mem.alloc(&f, sizeof(W) * (size_t)std::max(nbr, 1) * 3 + 64, "gbo_bfs");
-  if (rc) return rc;
+  int rc;
+  if (ext) {
+    f = (W*)ext->words;
+  } else if ((rc = mem.alloc(&f, sizeof(W) * (size_t)std::max(nbr, 1) * 3 + 64, "gbo_bfs"))) {
+    return rc;
+  }
   W* nx = f + std::max(nbr, 1);
   W* vis = nx + std::max(nbr, 1);
   int32_t* segoff = nullptr;
   if (Aout && (rc = mem.alloc(&segoff, sizeof(int32_t) * ((size_t)nbr + 1), "gbo_bfs_do"))) return rc;
   PlanGuard g(st);
   if ((rc = acquire_plan(A, &g.pv, st))) return rc;
-  const PlanView& pv = g.pv;
+  PlanView pv = g.pv;
+  if (ext) {
+    pv.slots = ext->slots;
+    pv.count = ext->count;
+    pv.bfs_counts = ext->dense_counts;
+    GBO_HIP_CHECK(hipMemsetAsync(pv.slots, 0, sizeof(unsigned long long) * kCountSlots, st));
+  }
   int64_t* d_cnt = pv.count;
   const unsigned blocks_n = (unsigned)((std::max<int64_t>(n, nbr) + 255) / 256);
   k_bfs_init<BS><<<blocks_n, 256, 0, st>>>(f, vis, d_level, nbr, n, source);
@@ -368,6 +391,57 @@ static int check_square(const gbo_bsr* A, const char* who) {
   return GBO_OK;
 }
 
+// ------------------------------------------------- allocation-free BFS
+// gbo_bfs_noalloc's scratch: one fixed array in device memory, held by
+// every GPU the library runs on until the process ends.  Host code never
+// passes `scratch` itself (in host code that name is only a stand-in); it
+// uses the device address hipGetSymbolAddress returns, looked up once per
+// device on the first call with that device current.
+constexpr size_t kNoAllocScratchBytes = (size_t)32 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[64];  // device address of `scratch`, per device
+
+static int noalloc_scratch(unsigned char** out) {
+  int dev = 0;
+  GBO_HIP_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) return set_error(GBO_EDEVICE, "gbo_bfs_noalloc: device %d out of range", dev);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return set_error(GBO_EDEVICE, "gbo_bfs_noalloc: hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+    g_scratch_ptr[dev] = p;
+  }
+  *out = (unsigned char*)g_scratch_ptr[dev];
+  return GBO_OK;
+}
+
+static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// Carves the BFS scratch for AT out of the static array; GBO_ERANGE when it
+// does not fit.
+static int noalloc_layout(const gbo_bsr* AT, unsigned char* base, BfsScratch* s) {
+  const size_t wbytes = (size_t)(AT->bs / 8) * (size_t)std::max(AT->nblockrows, 1) * 3 + 64;
+  const bool dense = AT->bs == 8 && AT->plan_variant == 3;
+  const size_t dbytes = dense ? sizeof(int32_t) * 8 * ((size_t)AT->nblockrows + 1) : 0;
+  const size_t o_count = align256(sizeof(unsigned long long) * kCountSlots);
+  const size_t o_words = o_count + 256;
+  const size_t o_dense = o_words + align256(wbytes);
+  const size_t need = o_dense + align256(dbytes);
+  if (need > kNoAllocScratchBytes)
+    return set_error(GBO_ERANGE,
+                     "gbo_bfs_noalloc: %d vertices at bs %d need %zu bytes of scratch, the static array holds %zu",
+                     AT->nrows, AT->bs, need, kNoAllocScratchBytes);
+  s->slots = (unsigned long long*)base;
+  s->count = (int64_t*)(base + o_count);
+  s->words = base + o_words;
+  s->dense_counts = dense ? (int32_t*)(base + o_dense) : nullptr;
+  return GBO_OK;
+}
+
 }  // namespace gbo
 
 using namespace gbo;
@@ -387,6 +461,30 @@ int gbo_bfs(const gbo_bsr* AT, int32_t source, int32_t* d_level, int64_t* h_leve
     case 64: return bfs_t<64>(AT, nullptr, source, d_level, h_level_sizes, max_levels, h_nlevels, stream);
   }
   return set_error(GBO_EUNSUPPORTED, "gbo_bfs: block size");
+}
+
+int gbo_bfs_noalloc(const gbo_bsr* AT, int32_t source, int32_t* d_level, int64_t* h_level_sizes,
+                    int32_t max_levels, int32_t* h_nlevels, gbo_stream_t stream) {
+  GBO_TRY_BEGIN
+  int rc = check_square(AT, "gbo_bfs_noalloc");
+  if (rc) return rc;
+  if (source < 0 || source >= AT->nrows)
+    return set_error(GBO_EINVAL, "gbo_bfs_noalloc: source %d out of range", source);
+  if (!d_level) return set_error(GBO_EINVAL, "gbo_bfs_noalloc: null level");
+  if (!AT->plan || AT->plan_chunk <= 0 || AT->plan_nchunks <= 0)
+    return set_error(GBO_EINVAL, "gbo_bfs_noalloc: the matrix has no plan (make one with gbo_bsr_plan first)");
+  unsigned char* base = nullptr;
+  if ((rc = noalloc_scratch(&base))) return rc;
+  BfsScratch s{};
+  if ((rc = noalloc_layout(AT, base, &s))) return rc;
+  switch (AT->bs) {
+    case 8: return bfs_t<8>(AT, nullptr, source, d_level, h_level_sizes, max_levels, h_nlevels, stream, &s);
+    case 16: return bfs_t<16>(AT, nullptr, source, d_level, h_level_sizes, max_levels, h_nlevels, stream, &s);
+    case 32: return bfs_t<32>(AT, nullptr, source, d_level, h_level_sizes, max_levels, h_nlevels, stream, &s);
+    case 64: return bfs_t<64>(AT, nullptr, source, d_level, h_level_sizes, max_levels, h_nlevels, stream, &s);
+  }
+  return set_error(GBO_EUNSUPPORTED, "gbo_bfs_noalloc: block size");
+  GBO_TRY_END
 }
 
 int gbo_bfs_do(const gbo_bsr* AT, const gbo_bsr* A, int32_t source, int32_t* d_level, int64_t* h_level_sizes,

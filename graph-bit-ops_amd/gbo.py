@@ -134,6 +134,7 @@ def _declare(L):
         "gbo_bsrbmm": ([POINTER(BsrStruct), POINTER(BsrStruct), POINTER(BsrStruct), P], c_int),
         "gbo_bsr_bit_offsets": ([POINTER(BsrStruct), P, i64p, P], c_int),
         "gbo_bfs": ([POINTER(BsrStruct), c_int32, P, P, c_int32, i32p, P], c_int),
+        "gbo_bfs_noalloc": ([POINTER(BsrStruct), c_int32, P, P, c_int32, i32p, P], c_int),
         "gbo_bfs_do": ([POINTER(BsrStruct), POINTER(BsrStruct), c_int32, P, P, c_int32, i32p, P], c_int),
         "gbo_sssp": ([POINTER(BsrStruct), P, P, c_int32, P, i32p, P], c_int),
         "gbo_pagerank": ([POINTER(BsrStruct), P, c_float, c_double, c_int32, P, i32p, POINTER(c_double), P], c_int),

@@ -330,6 +330,20 @@ int gbo_bsr_bit_offsets(const gbo_bsr* A, int64_t* d_off, int64_t* nnz, gbo_stre
 int gbo_bfs(const gbo_bsr* AT, int32_t source, int32_t* d_level,
             int64_t* h_level_sizes, int32_t max_levels, int32_t* h_nlevels,
             gbo_stream_t stream);
+/* Allocation-free gbo_bfs.  No counterpart in the reference: for callers that
+ * cannot allocate device memory during the call (serving loops, harnesses
+ * that forbid hipMalloc).  Same arguments and results as gbo_bfs (pull BFS
+ * over AT); AT must already have a plan (gbo_bsr_plan), else GBO_EINVAL.
+ * The frontier / next / visited words, the frontier-size counters and (bs 8,
+ * variant 3) the dense step's row counts live in one static device array of
+ * 32 MiB held by each GPU the library runs on until the process ends; a
+ * graph whose scratch exceeds it is refused with GBO_ERANGE before any
+ * launch (bs 8: up to about 7.6M vertices; bs 16-64: up to about 89M).
+ * Calls that overlap on one GPU share that array and race: order them (one
+ * stream, or synchronise between calls). */
+int gbo_bfs_noalloc(const gbo_bsr* AT, int32_t source, int32_t* d_level,
+                    int64_t* h_level_sizes, int32_t max_levels, int32_t* h_nlevels,
+                    gbo_stream_t stream);
 /* Direction-optimizing BFS, same outputs as gbo_bfs.  A is the out-edge
  * matrix (pass AT again for an undirected graph; NULL = pull only): a level
  * whose frontier holds at most n/64 vertices is pushed along the frontier
