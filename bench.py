@@ -180,6 +180,32 @@ def hbm_ceilings(gbo, timer, nbytes=1 << 30, reps=10):
     return out
 
 
+def gather_probes(gbo, A, dx, d_ci, nnz, n, timer, reps=10):
+    """Random-gather rate of this GPU for the two gather patterns of the
+    sweeps, each fed with the sweep's own indices and nothing else:
+    - count: one x byte per tile, indexed by the tiles' block columns (the
+      count sweep's x gathers plus its colind stream, without the tiles);
+    - value: one 4-byte value of an n-long vector per nonzero, indexed by
+      the CSR column of every nonzero (the value sweeps' gathers)."""
+    L = gbo.lib()
+    sink = gbo.DeviceBuffer(16)
+    vec = gbo.DeviceBuffer.from_numpy(np.full(n, 1.0 / n, np.float32))
+    out = {}
+    for name, table, elem, idx, cnt in (("count_x_gathers", dx.ptr, 1, A.s.colind, A.nblocks),
+                                        ("value_gathers", vec.ptr, 4, d_ci.ptr, nnz)):
+        fn = lambda: L.gbo_gather_probe(table, elem, idx, cnt, 0xFFFFFFFF, sink.ptr, None)
+        gbo._check(fn())
+        gbo.synchronize()
+        timer.start()
+        for _ in range(reps):
+            gbo._check(fn())
+        ms = timer.stop_ms() / reps
+        out[name] = {"gathers": int(cnt), "ms": round(ms, 4), "G_per_s": round(cnt / (ms * 1e-3) / 1e9, 1)}
+    sink.free()
+    vec.free()
+    return out
+
+
 def bench_multisource(gbo, A, g, timer, nsrc=256, seed=3):
     """C4a: 256-source batched BFS vs 256 plain BFS runs on the same GPU, all
     results in device memory (the plain runs write their levels into their
@@ -651,6 +677,8 @@ def main():
     best = min(per_bs.values(), key=lambda r: r["ms_per_sweep"])
     bs = best["bs"]
     A, dx = keep[bs]
+    probes = gather_probes(gbo, A, dx, d_ci, nnz, n, timer)
+    log(f"[bench] gather probes (bs {bs}): {probes}")
     extra = {}
     if not args.no_batch:
         for name, fn in (("multisource_bfs", lambda: bench_multisource(gbo, A, g, timer)),
@@ -686,7 +714,9 @@ def main():
                      "frac": best["frac_hbm"], "traffic": measured_traffic(bs, best["variant"], args.scale),
                      "algorithmic_bytes": best["bytes_per_sweep"], "copy_ceiling_GBs": ceil["copy"],
                      "read_ceiling_GBs": ceil["read"],
-                     "frac_of_read_ceiling": round(best["achieved_GBs"] / ceil["read"], 4)},
+                     "frac_of_read_ceiling": round(best["achieved_GBs"] / ceil["read"], 4),
+                     "gather_probe": probes,
+                     "sweep_over_gather_probe": round(best["ms_per_sweep"] / probes["count_x_gathers"]["ms"], 3)},
         "cpu_baseline": cpu,
         "per_block_size": per_bs,
         "triangle_counting": tc,

@@ -41,6 +41,16 @@ struct TcArgs {
   const int32_t* bt_tile;  //   tile index in B of each entry
   const void* b_tiles;
   unsigned long long* count;
+  // heavy block rows of A (many tiles): a bitmap of the row's block columns
+  // and the rank of each 64-bit word, so membership of J in the row and
+  // J's tile index are two loads instead of a binary search
+  const int32_t* hslot;    // per block row: slot of its bitmap, or -1 (nullable)
+  const uint64_t* hbits;   // [slots][hwords]
+  const int32_t* hrank;    // [slots][hwords]: set bits in the words before
+  int64_t hwords;
+  // per block column K of B: the slot whose bitmap holds B's column K list
+  // (B = L^T, so column K of B is row K of L), or -1 (nullable)
+  const int32_t* hcol;
 };
 
 // First index in [lo, hi) of sorted arr whose value is >= key.
@@ -135,13 +145,23 @@ __global__ __launch_bounds__(256) void k_tc(const TcArgs a) {
       const int64_t rs = __builtin_amdgcn_readlane(rsl, j), re = __builtin_amdgcn_readlane(rel, j);
       const int64_t s = rs > cb ? rs : cb;
       const int64_t te = re < ce ? re : ce;
+      const int32_t hs = a.hslot ? a.hslot[R0 + j] : -1;  // wave-uniform
+      const uint64_t* __restrict__ hb = hs >= 0 ? a.hbits + (int64_t)hs * a.hwords : nullptr;
+      const int32_t* __restrict__ hr = hs >= 0 ? a.hrank + (int64_t)hs * a.hwords : nullptr;
       for (int64_t tm = s; tm < te; ++tm) {  // mask tile (I, K)
         const int32_t K = a.a_colind[tm];
         // Intersect block row I of A (J ascending) with block column K of B;
         // every lane that finds a match adds that (I,J,K) term itself.
         const int64_t la0 = rs, la1 = re;
         const int64_t lb0 = a.bt_ptr[K], lb1 = a.bt_ptr[K + 1];
-        const bool a_short = (la1 - la0) <= (lb1 - lb0);
+        // a heavy row walks B's column and looks each J up in its bitmap; a
+        // short row facing a heavy column walks its own tiles and looks them
+        // up in the column's bitmap
+        const bool use_map = hs >= 0 && (lb1 - lb0) <= (la1 - la0);
+        const bool a_short = !use_map && (la1 - la0) <= (lb1 - lb0);
+        const int32_t hk = (a_short && a.hcol) ? a.hcol[K] : -1;
+        const uint64_t* __restrict__ cb_bits = hk >= 0 ? a.hbits + (int64_t)hk * a.hwords : nullptr;
+        const int32_t* __restrict__ cb_rank = hk >= 0 ? a.hrank + (int64_t)hk * a.hwords : nullptr;
         const int64_t s0 = a_short ? la0 : lb0, s1 = a_short ? la1 : lb1;
         const int64_t l0 = a_short ? lb0 : la0, l1 = a_short ? lb1 : la1;
         const int32_t* skeys = a_short ? a.a_colind : a.bt_row;
@@ -153,10 +173,24 @@ __global__ __launch_bounds__(256) void k_tc(const TcArgs a) {
           int64_t tA = -1, tB = -1;
           if (q < s1) {
             const int32_t J = skeys[q];
-            const int64_t pos = lower_bound_i32(lkeys, l0, l1, J);
-            if (pos < l1 && lkeys[pos] == J) {
-              tA = a_short ? q : pos;
-              tB = a_short ? a.bt_tile[pos] : a.bt_tile[q];
+            if (use_map) {
+              const uint64_t wd = hb[J >> 6];
+              if ((wd >> (J & 63)) & 1ull) {
+                tA = la0 + hr[J >> 6] + __popcll(wd & ((1ull << (J & 63)) - 1ull));
+                tB = a.bt_tile[q];
+              }
+            } else if (hk >= 0) {
+              const uint64_t wd = cb_bits[J >> 6];
+              if ((wd >> (J & 63)) & 1ull) {
+                tA = q;
+                tB = a.bt_tile[lb0 + cb_rank[J >> 6] + __popcll(wd & ((1ull << (J & 63)) - 1ull))];
+              }
+            } else {
+              const int64_t pos = lower_bound_i32(lkeys, l0, l1, J);
+              if (pos < l1 && lkeys[pos] == J) {
+                tA = a_short ? q : pos;
+                tB = a_short ? a.bt_tile[pos] : a.bt_tile[q];
+              }
             }
           }
           if constexpr (BS <= 16) {
@@ -199,6 +233,71 @@ __global__ __launch_bounds__(256) void k_tc(const TcArgs a) {
   }
   for (int off = 1; off < kWave; off <<= 1) total += __shfl_xor(total, off);
   if (lane == 0 && total) atomicAdd(a.count, (unsigned long long)total);
+}
+
+// Bitmaps of the heavy rows: block column J of row rows[h] sets bit J of
+// slot h (one workgroup per heavy row, its tiles taken strided).
+__global__ __launch_bounds__(256) void k_heavy_bits(const int32_t* __restrict__ rowptr,
+                                                    const int32_t* __restrict__ colind,
+                                                    const int32_t* __restrict__ rows, int64_t hwords,
+                                                    unsigned long long* __restrict__ bits) {
+  const int32_t I = rows[blockIdx.x];
+  unsigned long long* b = bits + (int64_t)blockIdx.x * hwords;
+  for (int64_t t = rowptr[I] + threadIdx.x; t < rowptr[I + 1]; t += blockDim.x) {
+    const int32_t J = colind[t];
+    atomicOr(&b[J >> 6], 1ull << (J & 63));
+  }
+}
+
+// hcol[K] = h when B's column K list (bt_row over [bt_ptr[K], bt_ptr[K+1]))
+// equals row K = rows[h] of A, so that slot h's bitmap and ranks also
+// locate J inside B's column K (one workgroup per slot).
+__global__ __launch_bounds__(256) void k_heavy_colcheck(const int32_t* __restrict__ rowptr,
+                                                        const int32_t* __restrict__ colind,
+                                                        const int32_t* __restrict__ bt_ptr,
+                                                        const int32_t* __restrict__ bt_row,
+                                                        const int32_t* __restrict__ rows, int32_t* __restrict__ hcol) {
+  __shared__ int same;
+  const int32_t K = rows[blockIdx.x];
+  const int32_t a0 = rowptr[K], n = rowptr[K + 1] - a0, b0 = bt_ptr[K];
+  if (threadIdx.x == 0) same = (bt_ptr[K + 1] - b0 == n);
+  __syncthreads();
+  if (same) {
+    for (int32_t i = threadIdx.x; i < n; i += blockDim.x)
+      if (colind[a0 + i] != bt_row[b0 + i]) same = 0;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0 && same) hcol[K] = (int32_t)blockIdx.x;
+}
+
+// rank[w] = set bits of the slot's words before w (one workgroup per slot:
+// each thread sums a contiguous run of words, a block scan of the 256 run
+// totals, then the runs are written out).
+__global__ __launch_bounds__(256) void k_heavy_rank(const unsigned long long* __restrict__ bits, int64_t hwords,
+                                                    int32_t* __restrict__ rank) {
+  __shared__ int32_t part[256];
+  const unsigned long long* b = bits + (int64_t)blockIdx.x * hwords;
+  int32_t* r = rank + (int64_t)blockIdx.x * hwords;
+  const int64_t per = (hwords + 255) / 256;
+  const int64_t w0 = min((int64_t)threadIdx.x * per, hwords), w1 = min(w0 + per, hwords);
+  int32_t s = 0;
+  for (int64_t w = w0; w < w1; ++w) s += __popcll(b[w]);
+  part[threadIdx.x] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int32_t acc = 0;
+    for (int k = 0; k < 256; ++k) {
+      const int32_t v = part[k];
+      part[k] = acc;
+      acc += v;
+    }
+  }
+  __syncthreads();
+  int32_t acc = part[threadIdx.x];
+  for (int64_t w = w0; w < w1; ++w) {
+    r[w] = acc;
+    acc += __popcll(b[w]);
+  }
 }
 
 // colsum (rows=false): S[J*BS + c] += #rows of tile (I,J) with column c set.
@@ -366,6 +465,58 @@ int gbo_bsrbmm_tc(const gbo_bsr* L, const gbo_bsr* LT, uint64_t* d_count, gbo_st
   a.bt_tile = bt_tile;
   a.b_tiles = rows;
   a.count = (unsigned long long*)d_count;
+  // heavy rows (more than kHeavyTiles tiles) get a block-column bitmap, the
+  // largest rows first, within a budget of device memory
+  {
+    constexpr int32_t kHeavyTiles = 64;
+    constexpr int64_t kHeavyBudget = (int64_t)1 << 30;
+    const int32_t nbr = L->nblockrows;
+    const int64_t hwords = ((int64_t)L->nblockcols + 63) / 64;
+    std::vector<int32_t> hrp((size_t)nbr + 1);
+    GBO_HIP_CHECK(hipMemcpyAsync(hrp.data(), L->rowptr, sizeof(int32_t) * hrp.size(), hipMemcpyDeviceToHost, st));
+    GBO_HIP_CHECK(hipStreamSynchronize(st));
+    std::vector<int32_t> heavy;
+    for (int32_t I = 0; I < nbr; ++I)
+      if (hrp[(size_t)I + 1] - hrp[(size_t)I] > kHeavyTiles) heavy.push_back(I);
+    std::sort(heavy.begin(), heavy.end(), [&](int32_t x, int32_t y) {
+      return hrp[(size_t)x + 1] - hrp[(size_t)x] > hrp[(size_t)y + 1] - hrp[(size_t)y];
+    });
+    const int64_t per_row = hwords * (int64_t)(sizeof(uint64_t) + sizeof(int32_t));
+    const size_t nh = (size_t)std::min<int64_t>((int64_t)heavy.size(), kHeavyBudget / std::max<int64_t>(per_row, 1));
+    heavy.resize(nh);
+    if (nh > 0) {
+      std::vector<int32_t> slot((size_t)nbr, -1);
+      for (size_t h = 0; h < nh; ++h) slot[(size_t)heavy[h]] = (int32_t)h;
+      int32_t *d_slot = nullptr, *d_rows = nullptr, *d_rank = nullptr;
+      unsigned long long* d_bits = nullptr;
+      if ((rc = mem.alloc(&d_slot, sizeof(int32_t) * (size_t)nbr, "gbo_bsrbmm_tc")) ||
+          (rc = mem.alloc(&d_rows, sizeof(int32_t) * nh, "gbo_bsrbmm_tc")) ||
+          (rc = mem.alloc(&d_bits, sizeof(uint64_t) * nh * (size_t)hwords, "gbo_bsrbmm_tc")) ||
+          (rc = mem.alloc(&d_rank, sizeof(int32_t) * nh * (size_t)hwords, "gbo_bsrbmm_tc")))
+        return rc;
+      GBO_HIP_CHECK(hipMemcpyAsync(d_slot, slot.data(), sizeof(int32_t) * (size_t)nbr, hipMemcpyHostToDevice, st));
+      GBO_HIP_CHECK(hipMemcpyAsync(d_rows, heavy.data(), sizeof(int32_t) * nh, hipMemcpyHostToDevice, st));
+      GBO_HIP_CHECK(hipMemsetAsync(d_bits, 0, sizeof(uint64_t) * nh * (size_t)hwords, st));
+      k_heavy_bits<<<(unsigned)nh, 256, 0, st>>>(L->rowptr, L->colind, d_rows, hwords, d_bits);
+      GBO_LAUNCH_CHECK();
+      k_heavy_rank<<<(unsigned)nh, 256, 0, st>>>(d_bits, hwords, d_rank);
+      GBO_LAUNCH_CHECK();
+      // L is square, so block column K of B = L^T is block row K of L: the
+      // row bitmaps serve B's heavy columns once the lists are checked equal
+      int32_t* d_hcol = nullptr;
+      if ((rc = mem.alloc(&d_hcol, sizeof(int32_t) * (size_t)nbr, "gbo_bsrbmm_tc"))) return rc;
+      GBO_HIP_CHECK(hipMemsetAsync(d_hcol, 0xFF, sizeof(int32_t) * (size_t)nbr, st));
+      k_heavy_colcheck<<<(unsigned)nh, 256, 0, st>>>(L->rowptr, L->colind, bt_ptr, bt_row, d_rows, d_hcol);
+      GBO_LAUNCH_CHECK();
+      a.hcol = d_hcol;
+      // the host vectors above are sources of asynchronous copies
+      GBO_HIP_CHECK(hipStreamSynchronize(st));
+      a.hslot = d_slot;
+      a.hbits = (const uint64_t*)d_bits;
+      a.hrank = d_rank;
+      a.hwords = hwords;
+    }
+  }
   const unsigned blocks = (unsigned)(((int64_t)a.nchunks + 3) / 4);
   switch (L->bs) {
     case 8: k_tc<8><<<blocks, 256, 0, st>>>(a); break;

@@ -49,9 +49,47 @@ __global__ __launch_bounds__(256) void k_stream_read(const u32x4* __restrict__ s
   if (x == never) sink[0] = x;
 }
 
+// Gather probe: out-of-order reads table[idx[i]] for i in [0, n), idx
+// streamed with 16-byte loads (four indices per lane), the four gathers of
+// a lane in flight together, every value XOR-folded per thread and stored
+// only when the fold equals `never`.  ELEM is the element size (1 or 4).
+template <int ELEM>
+__global__ __launch_bounds__(256) void k_gather_probe(const void* __restrict__ table, const int32_t* __restrict__ idx,
+                                                      int64_t n, uint32_t never, uint32_t* __restrict__ sink) {
+  typedef int i32x4 __attribute__((ext_vector_type(4)));
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  uint32_t x = 0;
+  const int64_t n4 = n / 4;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride) {
+    const i32x4 k = __builtin_nontemporal_load((const i32x4*)idx + i);
+    if constexpr (ELEM == 1) {
+      const uint8_t* t = (const uint8_t*)table;
+      x ^= (uint32_t)t[k.x] ^ ((uint32_t)t[k.y] << 8) ^ ((uint32_t)t[k.z] << 16) ^ ((uint32_t)t[k.w] << 24);
+    } else {
+      const uint32_t* t = (const uint32_t*)table;
+      x ^= t[k.x] ^ t[k.y] ^ t[k.z] ^ t[k.w];
+    }
+  }
+  for (int64_t i = 4 * n4 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    x ^= ELEM == 1 ? (uint32_t)((const uint8_t*)table)[idx[i]] : ((const uint32_t*)table)[idx[i]];
+  if (x == never) sink[0] = x;
+}
+
 }  // namespace
 
 extern "C" {
+
+int gbo_gather_probe(const void* d_table, int elem_bytes, const int32_t* d_idx, int64_t n, uint32_t never,
+                     uint32_t* d_sink, gbo_stream_t stream) {
+  if (!d_table || !d_idx || !d_sink || n < 0 || (elem_bytes != 1 && elem_bytes != 4) || (uintptr_t)d_idx % 16)
+    return gbo::set_error(GBO_EINVAL, "gbo_gather_probe: bad argument (elem_bytes 1 or 4, idx 16-byte aligned)");
+  if (n == 0) return GBO_OK;
+  const int64_t blocks = std::min<int64_t>((n / 4 + 255) / 256 + 1, 256 * 64);
+  if (elem_bytes == 1) k_gather_probe<1><<<(unsigned)blocks, 256, 0, stream>>>(d_table, d_idx, n, never, d_sink);
+  else k_gather_probe<4><<<(unsigned)blocks, 256, 0, stream>>>(d_table, d_idx, n, never, d_sink);
+  GBO_LAUNCH_CHECK();
+  return GBO_OK;
+}
 
 int gbo_stream_copy(void* d_dst, const void* d_src, size_t bytes, gbo_stream_t stream) {
   if (bytes % 16 || ((uintptr_t)d_dst | (uintptr_t)d_src) % 16)

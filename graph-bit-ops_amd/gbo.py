@@ -97,6 +97,7 @@ def _declare(L):
         "gbo_free_host": ([P], None),
         "gbo_stream_copy": ([P, P, c_size_t, P], c_int),
         "gbo_stream_read": ([P, c_size_t, c_uint32, P, P], c_int),
+        "gbo_gather_probe": ([P, c_int, P, c_int64, c_uint32, P, P], c_int),
         "gbo_mtx_read": ([c_char_p, c_int, c_int, i32p, i32p, i64p, POINTER(P), POINTER(P), POINTER(P)], c_int),
         "gbo_mtx_read_cached": ([c_char_p, c_int, c_int, POINTER(c_int), i32p, i32p, i64p, POINTER(P), POINTER(P),
                                  POINTER(P)], c_int),

@@ -107,6 +107,13 @@ int gbo_stream_copy(void* d_dst, const void* d_src, size_t bytes, gbo_stream_t s
  * 0xFFFFFFFF over a buffer of equal bytes), so nothing is written in
  * practice.  Measures the HBM read ceiling.  No reference counterpart. */
 int gbo_stream_read(const void* d_src, size_t bytes, uint32_t never, uint32_t* d_sink, gbo_stream_t stream);
+/* Gather probe: reads table[idx[i]] for i in [0, n) (elements of 1 or 4
+ * bytes; idx int32, 16-byte aligned, every index inside the table), four
+ * gathers per lane in flight, XOR-folded as in gbo_stream_read.  Measures
+ * the random-gather rate a product's vector gathers meet on this GPU (fed
+ * with the product's own column indices).  No reference counterpart. */
+int gbo_gather_probe(const void* d_table, int elem_bytes, const int32_t* d_idx, int64_t n, uint32_t never,
+                     uint32_t* d_sink, gbo_stream_t stream);
 
 /* ------------------------------------------------- Matrix Market readers --
  * semantics 0: mmio_info + mmio_data (algorithm/mmio_highlevel.h:7,146):
