@@ -246,9 +246,6 @@ inline bool parse_real(const char* p, const char** q, bool single, float* out) {
 int parse_entries(const char* d, const char* end, const MtxHeader& h, bool want_vals, bool single, Entries* e) {
   const int F = (h.field == 'P') ? 2 : (h.field == 'C') ? 4 : 3;
   const int64_t need = h.nnz * F;
-  e->r.resize((size_t)h.nnz);
-  e->c.resize((size_t)h.nnz);
-  if (want_vals) e->v.assign((size_t)h.nnz, 1.0f);
   const size_t len = (size_t)(end - d);
   const int64_t nrange =
       std::max<int64_t>(1, std::min<int64_t>((int64_t)omp_get_max_threads() * 8, (int64_t)(len >> 16) + 1));
@@ -264,6 +261,31 @@ int parse_entries(const char* d, const char* end, const MtxHeader& h, bool want_
   }
   for (int64_t i = 0; i < nrange; ++i) ntok[(size_t)i + 1] += ntok[(size_t)i];
   const int64_t total = ntok[(size_t)nrange];
+  // a short file is refused before the entry arrays are sized by its header
+  if (total < need) {
+    // the lowest malformed token still wins over the missing ones
+    for (int64_t i = 0; i < nrange && total < need; ++i) {
+      int64_t t = ntok[(size_t)i];
+      for (const char* p = b[(size_t)i]; p < b[(size_t)i + 1]; ++p) {
+        if (!starts(p)) continue;
+        const char* q = p;
+        long long xi;
+        float xf;
+        const int f = (int)(t % F);
+        const bool ok = f < 2 ? parse_int(p, &q, &xi) : parse_real(p, &q, single, &xf);
+        if (!ok) return set_error(GBO_EFORMAT, "malformed token in entry %lld of the mtx file", (long long)(t / F));
+        if (f < 2 && (xi < 1 || xi > (f == 0 ? h.m : h.n)))
+          return set_error(GBO_EFORMAT, "entry %lld out of bounds (%s index %lld, size %lld)", (long long)(t / F),
+                           f == 0 ? "row" : "column", xi, (long long)(f == 0 ? h.m : h.n));
+        ++t;
+      }
+    }
+    return set_error(GBO_EFORMAT, "not enough entries in mtx file (%lld of %lld)", (long long)(total / F),
+                     (long long)h.nnz);
+  }
+  e->r.resize((size_t)h.nnz);
+  e->c.resize((size_t)h.nnz);
+  if (want_vals) e->v.assign((size_t)h.nnz, 1.0f);
   std::vector<ParseErr> errs((size_t)nrange);
   const int64_t m = h.m, n = h.n;
 #pragma omp parallel for schedule(dynamic, 1)
@@ -311,9 +333,6 @@ int parse_entries(const char* d, const char* end, const MtxHeader& h, bool want_
     return set_error(GBO_EFORMAT, "entry %lld out of bounds (%s index %lld, size %lld)",
                      (long long)(first.tok / F), first.kind == 2 ? "row" : "column", first.val,
                      (long long)(first.kind == 2 ? m : n));
-  if (total < need)
-    return set_error(GBO_EFORMAT, "not enough entries in mtx file (%lld of %lld)", (long long)(total / F),
-                     (long long)h.nnz);
   return GBO_OK;
 }
 

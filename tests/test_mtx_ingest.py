@@ -283,3 +283,17 @@ def test_cache_not_writable(gbo, tmp_path):
     want = gbo.mtx_read(p, 1, 0)
     assert np.array_equal(c.colind, want.colind)
     assert not [f for f in os.listdir(tmp_path) if ".tmp" in f]  # no temporary left behind
+
+
+def test_short_file_with_huge_count_is_refused_cheaply(gbo, tmp_path):
+    """A header that declares far more entries than the file holds is refused
+    from the token count, before any entry array is sized by it."""
+    p = tmp_path / "short.mtx"
+    p.write_text("%%MatrixMarket matrix coordinate pattern general\n5 5 2000000000\n1 1\n2 3\n")
+    with pytest.raises(gbo.GboError) as e:
+        gbo.mtx_read(str(p), 1, 0)
+    assert e.value.code == -5 and "not enough entries" in str(e.value)
+    p.write_text("%%MatrixMarket matrix coordinate pattern general\n5 5 2000000000\n1 1\n9 3\n")
+    with pytest.raises(gbo.GboError) as e:
+        gbo.mtx_read(str(p), 0, 0)
+    assert "entry 1 out of bounds" in str(e.value)
