@@ -314,7 +314,7 @@ def bench_graph_batch(gbo, count=512, bs=None, seed=4):
             "plain_ms": round(tp * 1e3, 1), "speedup": round(tp / tb, 2), "bit_exact_vs_plain": exact}
 
 
-def bench_ppr(gbo, A, g, timer, K=64, nseeds=1000, seed=5, tol=1e-6, max_iter=200):
+def bench_ppr(gbo, A, g, timer, d_ci=None, K=64, nseeds=1000, seed=5, tol=1e-6, max_iter=200):
     """C5: PPR forward and adjoint, 64 Dirichlet(0.1) personalization
     vectors over 1000 seed vertices each; w ~ N(0, 1).  Inputs and outputs
     are device-resident ([n][K] float32); each solve is timed with HIP
@@ -341,6 +341,18 @@ def bench_ppr(gbo, A, g, timer, K=64, nseeds=1000, seed=5, tol=1e-6, max_iter=20
     ga, ita = gbo.ppr_adjoint_ptr(A, A, d_od.ptr, K, 0.85, d_s.ptr, d_p.ptr, d_w.ptr, d_g.ptr, tol, max_iter,
                                   strict=False)
     ta = timer.stop_ms()
+    probe = None
+    if d_ci is not None and K == 64:
+        # the SpMM's row reads alone: one 256-byte row of P per nonzero
+        sink = gbo.DeviceBuffer(16)
+        fn = lambda: gbo.lib().gbo_gather_probe(d_p.ptr, 256, d_ci.ptr, g.nnz, 0xFFFFFFFF, sink.ptr, None)
+        gbo._check(fn())
+        gbo.synchronize()
+        timer.start()
+        for _ in range(3):
+            gbo._check(fn())
+        probe = round(timer.stop_ms() / 3, 3)
+        sink.free()
     for b in (d_s, d_w, d_p, d_g, d_od):
         b.free()
     it_bytes = A.tile_bytes() + 4 * A.nblocks + 4 * (A.nblockrows + 1) + 3 * n * K * 4
@@ -349,7 +361,8 @@ def bench_ppr(gbo, A, g, timer, K=64, nseeds=1000, seed=5, tol=1e-6, max_iter=20
             "adjoint_ms": round(ta, 1), "adjoint_iters": ita, "adjoint_over_forward": round(ta / tf, 3),
             "ms_per_spmm_iter": round(ms_it, 3), "bytes_per_iter": int(it_bytes),
             "frac_hbm_per_iter": round(it_bytes / (ms_it * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-            "gathered_GBs": round(g.nnz * K * 4 / (ms_it * 1e-3) / 1e9, 1), "grad_alpha": ga}
+            "gathered_GBs": round(g.nnz * K * 4 / (ms_it * 1e-3) / 1e9, 1), "row_gather_probe_ms": probe,
+            "spmm_iter_over_probe": round(ms_it / probe, 3) if probe else None, "grad_alpha": ga}
 
 
 def cpu_baseline(gbo, A, xbits, n, target_s):
@@ -683,7 +696,7 @@ def main():
     if not args.no_batch:
         for name, fn in (("multisource_bfs", lambda: bench_multisource(gbo, A, g, timer)),
                          ("graph_batch", lambda: bench_graph_batch(gbo)),
-                         ("ppr", lambda: bench_ppr(gbo, keep_ppr(gbo, d_rp, d_ci, n), g, timer))):
+                         ("ppr", lambda: bench_ppr(gbo, keep_ppr(gbo, d_rp, d_ci, n), g, timer, d_ci))):
             try:
                 t = time.time()
                 extra[name] = fn()

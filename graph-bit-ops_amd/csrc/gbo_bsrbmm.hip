@@ -45,8 +45,8 @@ struct TcArgs {
   // and the rank of each 64-bit word, so membership of J in the row and
   // J's tile index are two loads instead of a binary search
   const int32_t* hslot;    // per block row: slot of its bitmap, or -1 (nullable)
-  const uint64_t* hbits;   // [slots][hwords]
-  const int32_t* hrank;    // [slots][hwords]: set bits in the words before
+  const uint4* hmap;       // [slots][hwords]: x, y = the word's 64 bits (low,
+                           // high), z = set bits in the words before it
   int64_t hwords;
   // per block column K of B: the slot whose bitmap holds B's column K list
   // (B = L^T, so column K of B is row K of L), or -1 (nullable)
@@ -146,8 +146,7 @@ __global__ __launch_bounds__(256) void k_tc(const TcArgs a) {
       const int64_t s = rs > cb ? rs : cb;
       const int64_t te = re < ce ? re : ce;
       const int32_t hs = a.hslot ? a.hslot[R0 + j] : -1;  // wave-uniform
-      const uint64_t* __restrict__ hb = hs >= 0 ? a.hbits + (int64_t)hs * a.hwords : nullptr;
-      const int32_t* __restrict__ hr = hs >= 0 ? a.hrank + (int64_t)hs * a.hwords : nullptr;
+      const uint4* __restrict__ hm = hs >= 0 ? a.hmap + (int64_t)hs * a.hwords : nullptr;
       for (int64_t tm = s; tm < te; ++tm) {  // mask tile (I, K)
         const int32_t K = a.a_colind[tm];
         // Intersect block row I of A (J ascending) with block column K of B;
@@ -160,8 +159,7 @@ __global__ __launch_bounds__(256) void k_tc(const TcArgs a) {
         const bool use_map = hs >= 0 && (lb1 - lb0) <= (la1 - la0);
         const bool a_short = !use_map && (la1 - la0) <= (lb1 - lb0);
         const int32_t hk = (a_short && a.hcol) ? a.hcol[K] : -1;
-        const uint64_t* __restrict__ cb_bits = hk >= 0 ? a.hbits + (int64_t)hk * a.hwords : nullptr;
-        const int32_t* __restrict__ cb_rank = hk >= 0 ? a.hrank + (int64_t)hk * a.hwords : nullptr;
+        const uint4* __restrict__ cm = hk >= 0 ? a.hmap + (int64_t)hk * a.hwords : nullptr;
         const int64_t s0 = a_short ? la0 : lb0, s1 = a_short ? la1 : lb1;
         const int64_t l0 = a_short ? lb0 : la0, l1 = a_short ? lb1 : la1;
         const int32_t* skeys = a_short ? a.a_colind : a.bt_row;
@@ -174,16 +172,18 @@ __global__ __launch_bounds__(256) void k_tc(const TcArgs a) {
           if (q < s1) {
             const int32_t J = skeys[q];
             if (use_map) {
-              const uint64_t wd = hb[J >> 6];
+              const uint4 e = hm[J >> 6];  // word and rank in one load
+              const uint64_t wd = (uint64_t)e.x | ((uint64_t)e.y << 32);
               if ((wd >> (J & 63)) & 1ull) {
-                tA = la0 + hr[J >> 6] + __popcll(wd & ((1ull << (J & 63)) - 1ull));
+                tA = la0 + (int64_t)e.z + __popcll(wd & ((1ull << (J & 63)) - 1ull));
                 tB = a.bt_tile[q];
               }
             } else if (hk >= 0) {
-              const uint64_t wd = cb_bits[J >> 6];
+              const uint4 e = cm[J >> 6];
+              const uint64_t wd = (uint64_t)e.x | ((uint64_t)e.y << 32);
               if ((wd >> (J & 63)) & 1ull) {
                 tA = q;
-                tB = a.bt_tile[lb0 + cb_rank[J >> 6] + __popcll(wd & ((1ull << (J & 63)) - 1ull))];
+                tB = a.bt_tile[lb0 + (int64_t)e.z + __popcll(wd & ((1ull << (J & 63)) - 1ull))];
               }
             } else {
               const int64_t pos = lower_bound_i32(lkeys, l0, l1, J);
@@ -236,16 +236,18 @@ __global__ __launch_bounds__(256) void k_tc(const TcArgs a) {
 }
 
 // Bitmaps of the heavy rows: block column J of row rows[h] sets bit J of
-// slot h (one workgroup per heavy row, its tiles taken strided).
+// slot h (one workgroup per heavy row, its tiles taken strided).  Each
+// 16-byte map entry holds a 64-bit word (x, y) and, after k_heavy_rank, the
+// set bits of the words before it (z).
 __global__ __launch_bounds__(256) void k_heavy_bits(const int32_t* __restrict__ rowptr,
                                                     const int32_t* __restrict__ colind,
                                                     const int32_t* __restrict__ rows, int64_t hwords,
-                                                    unsigned long long* __restrict__ bits) {
+                                                    uint4* __restrict__ map) {
   const int32_t I = rows[blockIdx.x];
-  unsigned long long* b = bits + (int64_t)blockIdx.x * hwords;
+  uint4* m = map + (int64_t)blockIdx.x * hwords;
   for (int64_t t = rowptr[I] + threadIdx.x; t < rowptr[I + 1]; t += blockDim.x) {
     const int32_t J = colind[t];
-    atomicOr(&b[J >> 6], 1ull << (J & 63));
+    atomicOr((unsigned long long*)&m[J >> 6], 1ull << (J & 63));
   }
 }
 
@@ -270,18 +272,16 @@ __global__ __launch_bounds__(256) void k_heavy_colcheck(const int32_t* __restric
   if (threadIdx.x == 0 && same) hcol[K] = (int32_t)blockIdx.x;
 }
 
-// rank[w] = set bits of the slot's words before w (one workgroup per slot:
-// each thread sums a contiguous run of words, a block scan of the 256 run
-// totals, then the runs are written out).
-__global__ __launch_bounds__(256) void k_heavy_rank(const unsigned long long* __restrict__ bits, int64_t hwords,
-                                                    int32_t* __restrict__ rank) {
+// rank (z) of every word = set bits of the slot's words before it (one
+// workgroup per slot: each thread sums a contiguous run of words, a block
+// scan of the 256 run totals, then the runs are written out).
+__global__ __launch_bounds__(256) void k_heavy_rank(int64_t hwords, uint4* __restrict__ map) {
   __shared__ int32_t part[256];
-  const unsigned long long* b = bits + (int64_t)blockIdx.x * hwords;
-  int32_t* r = rank + (int64_t)blockIdx.x * hwords;
+  uint4* m = map + (int64_t)blockIdx.x * hwords;
   const int64_t per = (hwords + 255) / 256;
   const int64_t w0 = min((int64_t)threadIdx.x * per, hwords), w1 = min(w0 + per, hwords);
   int32_t s = 0;
-  for (int64_t w = w0; w < w1; ++w) s += __popcll(b[w]);
+  for (int64_t w = w0; w < w1; ++w) s += __popc(m[w].x) + __popc(m[w].y);
   part[threadIdx.x] = s;
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -295,8 +295,9 @@ __global__ __launch_bounds__(256) void k_heavy_rank(const unsigned long long* __
   __syncthreads();
   int32_t acc = part[threadIdx.x];
   for (int64_t w = w0; w < w1; ++w) {
-    r[w] = acc;
-    acc += __popcll(b[w]);
+    const int32_t c = __popc(m[w].x) + __popc(m[w].y);
+    m[w].z = (uint32_t)acc;
+    acc += c;
   }
 }
 
@@ -481,25 +482,24 @@ int gbo_bsrbmm_tc(const gbo_bsr* L, const gbo_bsr* LT, uint64_t* d_count, gbo_st
     std::sort(heavy.begin(), heavy.end(), [&](int32_t x, int32_t y) {
       return hrp[(size_t)x + 1] - hrp[(size_t)x] > hrp[(size_t)y + 1] - hrp[(size_t)y];
     });
-    const int64_t per_row = hwords * (int64_t)(sizeof(uint64_t) + sizeof(int32_t));
+    const int64_t per_row = hwords * (int64_t)sizeof(uint4);
     const size_t nh = (size_t)std::min<int64_t>((int64_t)heavy.size(), kHeavyBudget / std::max<int64_t>(per_row, 1));
     heavy.resize(nh);
     if (nh > 0) {
       std::vector<int32_t> slot((size_t)nbr, -1);
       for (size_t h = 0; h < nh; ++h) slot[(size_t)heavy[h]] = (int32_t)h;
-      int32_t *d_slot = nullptr, *d_rows = nullptr, *d_rank = nullptr;
-      unsigned long long* d_bits = nullptr;
+      int32_t *d_slot = nullptr, *d_rows = nullptr;
+      uint4* d_map = nullptr;
       if ((rc = mem.alloc(&d_slot, sizeof(int32_t) * (size_t)nbr, "gbo_bsrbmm_tc")) ||
           (rc = mem.alloc(&d_rows, sizeof(int32_t) * nh, "gbo_bsrbmm_tc")) ||
-          (rc = mem.alloc(&d_bits, sizeof(uint64_t) * nh * (size_t)hwords, "gbo_bsrbmm_tc")) ||
-          (rc = mem.alloc(&d_rank, sizeof(int32_t) * nh * (size_t)hwords, "gbo_bsrbmm_tc")))
+          (rc = mem.alloc(&d_map, sizeof(uint4) * nh * (size_t)hwords, "gbo_bsrbmm_tc")))
         return rc;
       GBO_HIP_CHECK(hipMemcpyAsync(d_slot, slot.data(), sizeof(int32_t) * (size_t)nbr, hipMemcpyHostToDevice, st));
       GBO_HIP_CHECK(hipMemcpyAsync(d_rows, heavy.data(), sizeof(int32_t) * nh, hipMemcpyHostToDevice, st));
-      GBO_HIP_CHECK(hipMemsetAsync(d_bits, 0, sizeof(uint64_t) * nh * (size_t)hwords, st));
-      k_heavy_bits<<<(unsigned)nh, 256, 0, st>>>(L->rowptr, L->colind, d_rows, hwords, d_bits);
+      GBO_HIP_CHECK(hipMemsetAsync(d_map, 0, sizeof(uint4) * nh * (size_t)hwords, st));
+      k_heavy_bits<<<(unsigned)nh, 256, 0, st>>>(L->rowptr, L->colind, d_rows, hwords, d_map);
       GBO_LAUNCH_CHECK();
-      k_heavy_rank<<<(unsigned)nh, 256, 0, st>>>(d_bits, hwords, d_rank);
+      k_heavy_rank<<<(unsigned)nh, 256, 0, st>>>(hwords, d_map);
       GBO_LAUNCH_CHECK();
       // L is square, so block column K of B = L^T is block row K of L: the
       // row bitmaps serve B's heavy columns once the lists are checked equal
@@ -512,8 +512,7 @@ int gbo_bsrbmm_tc(const gbo_bsr* L, const gbo_bsr* LT, uint64_t* d_count, gbo_st
       // the host vectors above are sources of asynchronous copies
       GBO_HIP_CHECK(hipStreamSynchronize(st));
       a.hslot = d_slot;
-      a.hbits = (const uint64_t*)d_bits;
-      a.hrank = d_rank;
+      a.hmap = d_map;
       a.hwords = hwords;
     }
   }

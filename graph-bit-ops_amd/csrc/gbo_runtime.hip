@@ -75,15 +75,43 @@ __global__ __launch_bounds__(256) void k_gather_probe(const void* __restrict__ t
   if (x == never) sink[0] = x;
 }
 
+// Row-gather probe: a wave per index reads row idx[i] of a table of
+// 256-byte rows (lane l the l-th 4-byte word), eight rows in flight per
+// wave -- the row reads of a K = 64 SpMM.
+__global__ __launch_bounds__(256) void k_gather_rows(const uint32_t* __restrict__ table, const int32_t* __restrict__ idx,
+                                                     int64_t n, uint32_t never, uint32_t* __restrict__ sink) {
+  const int lane = threadIdx.x & 63;
+  const int64_t w = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  uint32_t x = 0;
+  for (int64_t i = w * 8; i < n; i += nw * 8) {
+    int32_t k[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) k[u] = i + u < n ? idx[i + u] : idx[i];
+    uint32_t v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = table[(int64_t)k[u] * 64 + lane];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) x ^= v[u];
+  }
+  if (x == never) sink[0] = x;
+}
+
 }  // namespace
 
 extern "C" {
 
 int gbo_gather_probe(const void* d_table, int elem_bytes, const int32_t* d_idx, int64_t n, uint32_t never,
                      uint32_t* d_sink, gbo_stream_t stream) {
-  if (!d_table || !d_idx || !d_sink || n < 0 || (elem_bytes != 1 && elem_bytes != 4) || (uintptr_t)d_idx % 16)
-    return gbo::set_error(GBO_EINVAL, "gbo_gather_probe: bad argument (elem_bytes 1 or 4, idx 16-byte aligned)");
+  if (!d_table || !d_idx || !d_sink || n < 0 || (elem_bytes != 1 && elem_bytes != 4 && elem_bytes != 256) ||
+      (uintptr_t)d_idx % 16)
+    return gbo::set_error(GBO_EINVAL, "gbo_gather_probe: bad argument (elem_bytes 1, 4 or 256, idx 16-byte aligned)");
   if (n == 0) return GBO_OK;
+  if (elem_bytes == 256) {
+    k_gather_rows<<<256 * 32, 256, 0, stream>>>((const uint32_t*)d_table, d_idx, n, never, d_sink);
+    GBO_LAUNCH_CHECK();
+    return GBO_OK;
+  }
   const int64_t blocks = std::min<int64_t>((n / 4 + 255) / 256 + 1, 256 * 64);
   if (elem_bytes == 1) k_gather_probe<1><<<(unsigned)blocks, 256, 0, stream>>>(d_table, d_idx, n, never, d_sink);
   else k_gather_probe<4><<<(unsigned)blocks, 256, 0, stream>>>(d_table, d_idx, n, never, d_sink);

@@ -109,9 +109,11 @@ int gbo_stream_copy(void* d_dst, const void* d_src, size_t bytes, gbo_stream_t s
 int gbo_stream_read(const void* d_src, size_t bytes, uint32_t never, uint32_t* d_sink, gbo_stream_t stream);
 /* Gather probe: reads table[idx[i]] for i in [0, n) (elements of 1 or 4
  * bytes; idx int32, 16-byte aligned, every index inside the table), four
- * gathers per lane in flight, XOR-folded as in gbo_stream_read.  Measures
- * the random-gather rate a product's vector gathers meet on this GPU (fed
- * with the product's own column indices).  No reference counterpart. */
+ * gathers per lane in flight, XOR-folded as in gbo_stream_read; with
+ * elem_bytes 256 a wave reads each 256-byte row idx[i] (lane l its l-th
+ * word), eight rows in flight.  Measures the random-gather rate a product's
+ * vector gathers meet on this GPU (fed with the product's own column
+ * indices).  No reference counterpart. */
 int gbo_gather_probe(const void* d_table, int elem_bytes, const int32_t* d_idx, int64_t n, uint32_t never,
                      uint32_t* d_sink, gbo_stream_t stream);
 
