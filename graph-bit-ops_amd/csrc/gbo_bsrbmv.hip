@@ -22,12 +22,12 @@ namespace gbo {
 
 // Tiles per group and round of the row-group min / sum loops: their words
 // and column indices are loaded together, then one set bit of each is
-// gathered per pass.  Eight measured faster than four for min at bs 32/64
-// and the sum at bs 64 (R-MAT s22: 3-6 %), slower for the sum at bs 32 and
-// far slower for the weighted min-plus loop (bs 32 4.9 -> 6.8 ms), which
-// keeps four.
+// gathered per pass.  Eight measured faster than four for min and sum at bs
+// 32/64 (R-MAT s22: 3-10 %; bs 32 sum 3.00 -> 2.69 ms once the round's
+// loads were unconditional) and far slower for the weighted min-plus loop
+// (bs 32 4.9 -> 6.8 ms), which keeps four.
 template <int BS, int OP>
-constexpr int kValueTiles = (BS == 64 || (BS == 32 && OP == OP_MIN)) ? 8 : 4;
+constexpr int kValueTiles = (BS >= 32) ? 8 : 4;
 
 template <int NU>
 __device__ __forceinline__ bool any_word(const uint64_t* wd) {
