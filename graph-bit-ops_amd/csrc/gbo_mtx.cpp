@@ -666,6 +666,24 @@ bool load_cache(const std::string& name, const CacheHeader& key, bool want_vals,
     }
     ok = ok && out->rp[0] == 0 && out->rp[h.m] == (int32_t)h.nnz;
   }
+  if (ok) {
+    // a damaged cache (rows out of order, columns out of range) is parsed
+    // again instead of reaching the packer
+    int bad = 0;
+    const int32_t* rp = out->rp;
+    const int32_t* ci = out->ci;
+    const int32_t m = h.m, n = h.n;
+#pragma omp parallel for schedule(static) reduction(| : bad)
+    for (int32_t r = 0; r < m; ++r) {
+      const int32_t s = rp[r], e = rp[r + 1];
+      if (s < 0 || s > e || e > (int32_t)h.nnz) {
+        bad = 1;
+        continue;
+      }
+      for (int32_t k = s; k < e; ++k) bad |= (ci[k] < 0 || ci[k] >= n) ? 1 : 0;
+    }
+    ok = !bad;
+  }
   ::close(fd);
   if (!ok) {
     std::free(out->rp);

@@ -275,6 +275,31 @@ def test_cache_invalidation(gbo, tmp_path, monkeypatch):
     assert gbo.mtx_read(p, 1, 0, cache=True).cache_state == 1
 
 
+@pytest.mark.parametrize("damage", ["column", "rowptr"])
+def test_cache_damaged_content_is_parsed_again(gbo, tmp_path, damage):
+    """A cache of the right key and size whose arrays do not form a CSR (a
+    column index out of range, a decreasing rowptr) is not handed out."""
+    p = _sym_file(tmp_path, np.random.default_rng(6))
+    want = gbo.mtx_read(p, 1, 0)
+    assert gbo.mtx_read(p, 1, 0, cache=True).cache_state == 0
+    name = gbo.mtx_cache_path(p, 1, 0)
+    hdr = 72  # magic, version, 3 flags, source size, mtime (s, ns), m, n, nnz, has_val, reserved
+    raw = bytearray(open(name, "rb").read())
+    has_val = int(np.frombuffer(bytes(raw[64:68]), np.uint32)[0])
+    assert len(raw) == hdr + 4 * (want.nrows + 1) + 4 * want.nnz * (2 if has_val else 1)
+    if damage == "column":
+        at = hdr + 4 * (want.nrows + 1) + 4 * (want.nnz // 2)
+        raw[at:at + 4] = np.int32(want.ncols).tobytes()
+    else:
+        r = int(np.argmax(np.diff(want.rowptr) > 0)) + 1  # rowptr[r] > rowptr[r - 1]
+        raw[hdr + 4 * r:hdr + 4 * r + 4] = np.int32(want.nnz).tobytes()
+    open(name, "r+b").write(bytes(raw))  # same size, same key
+    c = gbo.mtx_read(p, 1, 0, cache=True)
+    assert c.cache_state == 0
+    assert np.array_equal(c.rowptr, want.rowptr) and np.array_equal(c.colind, want.colind)
+    assert gbo.mtx_read(p, 1, 0, cache=True).cache_state == 1
+
+
 def test_cache_not_writable(gbo, tmp_path):
     p = _sym_file(tmp_path, np.random.default_rng(5))
     os.mkdir(gbo.mtx_cache_path(p, 1, 0))  # a directory in the cache's place
