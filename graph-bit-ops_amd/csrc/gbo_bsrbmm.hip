@@ -51,7 +51,19 @@ struct TcArgs {
   // per block column K of B: the slot whose bitmap holds B's column K list
   // (B = L^T, so column K of B is row K of L), or -1 (nullable)
   const int32_t* hcol;
+  // bs 32/64 (nullable): the set bits of every A tile, row-major (entry =
+  // row * BS + column), from aoff[t]; a match whose A tile has at most
+  // kLaneMatchBits of them is counted by its own lane
+  const int32_t* aoff;
+  const uint16_t* abits;
+  int vec16;  // bs 16: the tiles of A and B are 16-byte aligned (vector loads)
 };
+
+// A match (I,J,K) at bs 32/64 whose A tile (I,J) has at most this many set
+// bits is counted by one lane (a popcount per bit from the tile's bit list)
+// instead of by a group of bs lanes, one per tile row: R-MAT tiles hold 2-3
+// bits, so the group form left all but a few of its lanes idle.
+constexpr int32_t kLaneMatchBits = 16;
 
 // First index in [lo, hi) of sorted arr whose value is >= key.
 __device__ __forceinline__ int64_t lower_bound_i32(const int32_t* __restrict__ arr, int64_t lo, int64_t hi, int32_t key) {
@@ -90,8 +102,33 @@ __global__ __launch_bounds__(256) void k_tile_rows(const word_t<BS>* __restrict_
 // one popcount per set bit of A.
 template <int BS>
 __device__ __forceinline__ uint64_t tc_hit(const word_t<BS>* __restrict__ A, const word_t<BS>* __restrict__ M,
-                                           const word_t<BS>* __restrict__ R) {
+                                           const word_t<BS>* __restrict__ R, bool vec16) {
   uint64_t s = 0;
+  if constexpr (BS == 16) {
+    if (vec16) {
+    // A and M as four 64-bit words each (word q = rows 4q..4q+3, row 4q+i
+    // in bits 16i..16i+15), all loaded together, instead of a chain of
+    // row loads; R rows are gathered per set bit of A
+    typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+    const u32x4 a0 = *(const u32x4*)A, a1 = *((const u32x4*)A + 1);
+    const u32x4 m0 = *(const u32x4*)M, m1 = *((const u32x4*)M + 1);
+    const uint64_t aw[4] = {a0.x | ((uint64_t)a0.y << 32), a0.z | ((uint64_t)a0.w << 32),
+                            a1.x | ((uint64_t)a1.y << 32), a1.z | ((uint64_t)a1.w << 32)};
+    const uint64_t mw[4] = {m0.x | ((uint64_t)m0.y << 32), m0.z | ((uint64_t)m0.w << 32),
+                            m1.x | ((uint64_t)m1.y << 32), m1.z | ((uint64_t)m1.w << 32)};
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      uint64_t wa = aw[q];
+      while (wa) {
+        const int p = __builtin_ctzll(wa);
+        wa &= wa - 1;
+        const int i = p >> 4, col = 15 - (p & 15);  // row 4q+i, column col
+        s += __popc((uint32_t)((mw[q] >> (16 * i)) & 0xFFFFu) & (uint32_t)R[col]);
+      }
+    }
+    return s;
+    }
+  }
   if constexpr (BS == 8) {
     uint64_t a = *(const uint64_t*)A;  // byte r = row r, bit 7 - c = column c
     const uint64_t m = *(const uint64_t*)M, r = *(const uint64_t*)R;
@@ -117,8 +154,11 @@ __device__ __forceinline__ uint64_t tc_hit(const word_t<BS>* __restrict__ A, con
   return s;
 }
 
+// Eight waves per SIMD: the kernel is a chain of dependent loads, and
+// without the bound its scalar registers (TcArgs) cost one wave per SIMD
+// (a few of them move to vector-register lanes instead).
 template <int BS>
-__global__ __launch_bounds__(256) void k_tc(const TcArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_tc(const TcArgs a) {
   using W = word_t<BS>;
   const int lane = lane_id();
   const int c = wave_index();
@@ -195,11 +235,24 @@ __global__ __launch_bounds__(256) void k_tc(const TcArgs a) {
           }
           if constexpr (BS <= 16) {
             // each lane adds its own match
-            if (tA >= 0) total += tc_hit<BS>(At + tA * BS, At + tm * BS, Rt + tB * BS);
+            if (tA >= 0) total += tc_hit<BS>(At + tA * BS, At + tm * BS, Rt + tB * BS, a.vec16 != 0);
           } else {
             // 64/BS matches at a time, one per group of BS lanes (lane = row)
             constexpr int G = kWave / BS;
             const int grp = lane / BS, r = lane % BS;
+            // a sparse A tile: this lane counts the match alone, from the
+            // tile's bit list (mask row a, B row j per set bit (a, j))
+            if (tA >= 0 && a.abits) {
+              const int32_t o0 = a.aoff[tA], o1 = a.aoff[tA + 1];
+              if (o1 - o0 <= kLaneMatchBits) {
+                for (int32_t k = o0; k < o1; ++k) {
+                  const uint32_t e = a.abits[k];
+                  total += (uint64_t)popcw((W)((uint64_t)At[tm * BS + e / BS] & (uint64_t)Rt[tB * BS + e % BS]));
+                }
+                tA = -1;
+              }
+            }
+            // the other matches: 64/BS at a time, one per group of BS lanes
             uint64_t hits = __ballot(tA >= 0);
             while (hits) {
               int hsel = -1;
@@ -233,6 +286,45 @@ __global__ __launch_bounds__(256) void k_tc(const TcArgs a) {
   }
   for (int off = 1; off < kWave; off <<= 1) total += __shfl_xor(total, off);
   if (lane == 0 && total) atomicAdd(a.count, (unsigned long long)total);
+}
+
+// Set bits per tile (a group of BS lanes per tile, lane = word).
+template <int BS>
+__global__ __launch_bounds__(256) void k_tile_bitcount(const word_t<BS>* __restrict__ tiles, int64_t T,
+                                                       int32_t* __restrict__ cnt) {
+  constexpr int G = kWave / BS;
+  const int lane = lane_id();
+  const int64_t t = (int64_t)wave_index() * G + lane / BS;
+  int c = t < T ? popcw(tiles[t * BS + lane % BS]) : 0;
+  for (int off = 1; off < BS; off <<= 1) c += __shfl_xor(c, off);
+  if (lane % BS == 0 && t < T) cnt[t] = c;
+}
+
+// Set-bit list of every tile from off[t]: entries row * BS + column, rows in
+// order, columns ascending.
+template <int BS>
+__global__ __launch_bounds__(256) void k_tile_bitlist(const word_t<BS>* __restrict__ tiles, int64_t T,
+                                                      const int32_t* __restrict__ off, uint16_t* __restrict__ list) {
+  constexpr int G = kWave / BS;
+  const int lane = lane_id();
+  const int64_t t = (int64_t)wave_index() * G + lane / BS;
+  const int r = lane % BS;
+  const uint64_t w = t < T ? (uint64_t)tiles[t * BS + r] : 0ull;
+  const int pc = __popcll(w);
+  int before = pc;  // set bits of this tile's rows before r
+  for (int o = 1; o < BS; o <<= 1) {
+    const int v = __shfl_up(before, o, BS);
+    if (r >= o) before += v;
+  }
+  before -= pc;
+  if (t >= T) return;
+  int32_t at = off[t] + before;
+  uint64_t m = w;
+  while (m) {
+    const int p = 63 - __builtin_clzll(m);  // highest set bit: column BS-1-p
+    m &= ~(1ull << p);
+    list[at++] = (uint16_t)(r * BS + (BS - 1 - p));
+  }
 }
 
 // Bitmaps of the heavy rows: block column J of row rows[h] sets bit J of
@@ -446,26 +538,45 @@ int gbo_bsrbmm_tc(const gbo_bsr* L, const gbo_bsr* LT, uint64_t* d_count, gbo_st
     }
     GBO_LAUNCH_CHECK();
   }
-  // TC work per mask tile is far larger than a bsrbmv tile: small chunks
-  // (a plan without the bsrbmv scratch).
-  gbo_bsr tmp = *L;
-  tmp.plan = nullptr;
-  tmp.plan_chunk = 0;
-  if ((rc = plan_create(&tmp, 16, 0, false, st))) return rc;
   TcArgs a{};
   a.a_rowptr = L->rowptr;
   a.a_colind = L->colind;
   a.a_tiles = L->tiles;
   a.nbr = L->nblockrows;
   a.T = L->nblocks;
-  a.plan = tmp.plan;
-  a.chunk = tmp.plan_chunk;
-  a.nchunks = tmp.plan_nchunks;
   a.bt_ptr = bt_ptr;
   a.bt_row = bt_row;
   a.bt_tile = bt_tile;
   a.b_tiles = rows;
   a.count = (unsigned long long*)d_count;
+  a.vec16 = (((uintptr_t)L->tiles) & 15) == 0;
+  // bs 32/64: set bits per A tile and their scan; the total is read at the
+  // synchronization below, and the bit lists are built when the tiles are
+  // sparse (at most 8 set bits per tile on average)
+  int32_t* aoff = nullptr;
+  int32_t nbits = 0;
+  // copies into host memory (nbits, hrp) are waited for on every return
+  // until the explicit synchronization below
+  struct SyncOnExit {
+    hipStream_t st;
+    bool armed = false;
+    ~SyncOnExit() {
+      if (armed) (void)hipStreamSynchronize(st);
+    }
+  } pending{st};
+  std::vector<int32_t> hrp((size_t)L->nblockrows + 1);
+  if (L->bs >= 32) {
+    const int64_t T = L->nblocks;
+    if ((rc = mem.alloc(&aoff, sizeof(int32_t) * ((size_t)T + 1), "gbo_bsrbmm_tc"))) return rc;
+    const int64_t waves = (T * L->bs + kWave - 1) / kWave;
+    const unsigned gb = (unsigned)((waves + 3) / 4);
+    if (L->bs == 32) k_tile_bitcount<32><<<gb, 256, 0, st>>>((const uint32_t*)L->tiles, T, aoff);
+    else k_tile_bitcount<64><<<gb, 256, 0, st>>>((const uint64_t*)L->tiles, T, aoff);
+    GBO_LAUNCH_CHECK();
+    if ((rc = scan_exclusive<int32_t, int32_t>(aoff, aoff, T, st))) return rc;
+    pending.armed = true;
+    GBO_HIP_CHECK(hipMemcpyAsync(&nbits, aoff + T, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  }
   // heavy rows (more than kHeavyTiles tiles) get a block-column bitmap, the
   // largest rows first, within a budget of device memory
   {
@@ -473,9 +584,23 @@ int gbo_bsrbmm_tc(const gbo_bsr* L, const gbo_bsr* LT, uint64_t* d_count, gbo_st
     constexpr int64_t kHeavyBudget = (int64_t)1 << 30;
     const int32_t nbr = L->nblockrows;
     const int64_t hwords = ((int64_t)L->nblockcols + 63) / 64;
-    std::vector<int32_t> hrp((size_t)nbr + 1);
+    pending.armed = true;
     GBO_HIP_CHECK(hipMemcpyAsync(hrp.data(), L->rowptr, sizeof(int32_t) * hrp.size(), hipMemcpyDeviceToHost, st));
     GBO_HIP_CHECK(hipStreamSynchronize(st));
+    pending.armed = false;
+    if (aoff && (double)nbits <= 8.0 * (double)L->nblocks) {
+      const int64_t T = L->nblocks;
+      uint16_t* abits = nullptr;
+      if ((rc = mem.alloc(&abits, sizeof(uint16_t) * (size_t)std::max<int32_t>(nbits, 1), "gbo_bsrbmm_tc")))
+        return rc;
+      const int64_t waves = (T * L->bs + kWave - 1) / kWave;
+      const unsigned gb = (unsigned)((waves + 3) / 4);
+      if (L->bs == 32) k_tile_bitlist<32><<<gb, 256, 0, st>>>((const uint32_t*)L->tiles, T, aoff, abits);
+      else k_tile_bitlist<64><<<gb, 256, 0, st>>>((const uint64_t*)L->tiles, T, aoff, abits);
+      GBO_LAUNCH_CHECK();
+      a.aoff = aoff;
+      a.abits = abits;
+    }
     std::vector<int32_t> heavy;
     for (int32_t I = 0; I < nbr; ++I)
       if (hrp[(size_t)I + 1] - hrp[(size_t)I] > kHeavyTiles) heavy.push_back(I);
@@ -488,12 +613,14 @@ int gbo_bsrbmm_tc(const gbo_bsr* L, const gbo_bsr* LT, uint64_t* d_count, gbo_st
     if (nh > 0) {
       std::vector<int32_t> slot((size_t)nbr, -1);
       for (size_t h = 0; h < nh; ++h) slot[(size_t)heavy[h]] = (int32_t)h;
+      SyncOnExit sources{st};  // slot and heavy are sources of the copies below
       int32_t *d_slot = nullptr, *d_rows = nullptr;
       uint4* d_map = nullptr;
       if ((rc = mem.alloc(&d_slot, sizeof(int32_t) * (size_t)nbr, "gbo_bsrbmm_tc")) ||
           (rc = mem.alloc(&d_rows, sizeof(int32_t) * nh, "gbo_bsrbmm_tc")) ||
           (rc = mem.alloc(&d_map, sizeof(uint4) * nh * (size_t)hwords, "gbo_bsrbmm_tc")))
         return rc;
+      sources.armed = true;
       GBO_HIP_CHECK(hipMemcpyAsync(d_slot, slot.data(), sizeof(int32_t) * (size_t)nbr, hipMemcpyHostToDevice, st));
       GBO_HIP_CHECK(hipMemcpyAsync(d_rows, heavy.data(), sizeof(int32_t) * nh, hipMemcpyHostToDevice, st));
       GBO_HIP_CHECK(hipMemsetAsync(d_map, 0, sizeof(uint4) * nh * (size_t)hwords, st));
@@ -511,11 +638,22 @@ int gbo_bsrbmm_tc(const gbo_bsr* L, const gbo_bsr* LT, uint64_t* d_count, gbo_st
       a.hcol = d_hcol;
       // the host vectors above are sources of asynchronous copies
       GBO_HIP_CHECK(hipStreamSynchronize(st));
+      sources.armed = false;
       a.hslot = d_slot;
       a.hmap = d_map;
       a.hwords = hwords;
     }
   }
+  // TC work per mask tile is far larger than a bsrbmv tile: small chunks
+  // (a plan without the bsrbmv scratch), made last so that no early return
+  // above leaves it allocated
+  gbo_bsr tmp = *L;
+  tmp.plan = nullptr;
+  tmp.plan_chunk = 0;
+  if ((rc = plan_create(&tmp, 16, 0, false, st))) return rc;
+  a.plan = tmp.plan;
+  a.chunk = tmp.plan_chunk;
+  a.nchunks = tmp.plan_nchunks;
   const unsigned blocks = (unsigned)(((int64_t)a.nchunks + 3) / 4);
   switch (L->bs) {
     case 8: k_tc<8><<<blocks, 256, 0, st>>>(a); break;

@@ -203,6 +203,59 @@ def test_tc_vs_oracle(gbo, po, graph_cases, bs):
         assert gbo.tc(g, bs) == want, name
 
 
+def _clique_on_rmat(gbo, scale, ef, k, seed):
+    """R-MAT (sparse tiles) with a k-clique on vertices 0..k-1 (dense tiles)."""
+    from graphs import sym_csr
+    g = gbo.gen_rmat(scale, ef, seed=seed)
+    u = np.repeat(np.arange(g.nrows), np.diff(g.rowptr))
+    cu, cv = np.triu_indices(k, 1)
+    rp, ci = sym_csr(g.nrows, np.concatenate([u, cu]), np.concatenate([g.colind, cv]))
+    return gbo.Csr(g.nrows, g.nrows, rp, ci)
+
+
+@pytest.mark.parametrize("bs", (32, 64))
+def test_tc_lane_and_group_matches(gbo, po, bs):
+    """bs 32/64 count a match whose A tile has at most 16 set bits in one lane
+    (from the tile's bit list, built when the tiles average at most 8 bits)
+    and a denser one with a group of bs lanes.  An R-MAT graph with a 100-clique
+    has both kinds in one call; the king grid's tiles are dense (no lists)."""
+    g = _clique_on_rmat(gbo, 15, 4, 100, 3)
+    L = g.tril(strict=True)
+    Lr = gbo.BsrMatrix.from_csr(L, bs, 0)
+    _, _, tw = Lr.host_arrays()
+    per_tile = np.unpackbits(tw.view(np.uint8)).reshape(Lr.nblocks, -1).sum(axis=1)
+    assert per_tile.mean() <= 8 and (per_tile > 16).any() and (per_tile <= 16).any()
+    want = po.tc(g.nrows, g.rowptr, g.colind)
+    assert gbo.tc(g, bs) == want
+    king = gbo.gen_king(300)
+    assert gbo.tc(king, bs) == 4 * 299 ** 2
+
+
+def test_tc_bs16_unaligned_tiles(gbo, po):
+    """bs 16 reads A's tiles as 16-byte vectors when they are 16-byte aligned;
+    tiles at an address that is not (a view 2 bytes into a buffer) take the
+    row-by-row form and count the same."""
+    import ctypes
+    g = gbo.gen_rmat(11, 16, seed=5)
+    L = g.tril(strict=True)
+    Lr = gbo.BsrMatrix.from_csr(L, 16, 0)
+    Lc = gbo.BsrMatrix.from_csr(L.transpose(), 16, 1)
+    want = po.tc(g.nrows, g.rowptr, g.colind)
+    assert gbo.bsrbmm_tc(Lr, Lc) == want
+    nb = Lr.tile_bytes()
+    buf = gbo.DeviceBuffer(nb + 16)
+    gbo._check(gbo.lib().gbo_memcpy_d2d(buf.ptr + 2, Lr.s.tiles, nb))
+    s2 = gbo.BsrStruct()
+    ctypes.memmove(ctypes.addressof(s2), ctypes.addressof(Lr.s), ctypes.sizeof(s2))
+    s2.tiles = buf.ptr + 2
+    s2.plan = None
+    d = gbo.DeviceBuffer(8)
+    gbo._check(gbo.lib().gbo_bsrbmm_tc(ctypes.byref(s2), Lc.ref, d.ptr, None))
+    assert int(d.to_numpy(np.uint64, 1)[0]) == want
+    d.free()
+    buf.free()
+
+
 def test_tc_full_size_workloads(gbo, po):
     """The full-size triangle-counting workloads: the 2048x2048 king grid
     (closed form 4*2047^2 = 16,760,836) and R-MAT s20 (seed 2) against the
