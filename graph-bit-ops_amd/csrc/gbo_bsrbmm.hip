@@ -8,12 +8,14 @@
 // Here the block-column structure of B is transposed once (a stable radix
 // sort of the tiles by block column on the device: csr2csc on blocks) so
 // that the J's that contribute to mask tile (I,K) are the intersection of
-// two sorted lists: block row I of A and block column K of B.  Each wave takes a fixed chunk
-// of mask tiles; its 64 lanes search the longer list for 64 entries of the
-// shorter one at a time, and each lane that finds a match J adds that
-// term itself: with B's tiles turned into row form once per call,
+// two sorted lists: block row I of A and block column K of B.  Each wave
+// takes a fixed chunk of mask tiles; its 64 lanes look up 64 entries of the
+// shorter list at a time in the longer one (a binary search, or one load
+// from the bitmap of a heavy row's block columns), or, when both lists have
+// bitmaps, AND the two maps 64 words at a time.  Each match J adds
 // sum_{a,k} M[a][k] (A_IJ B_JK)[a][k] = sum over the set bits (a,j) of A_IJ
-// of popc(M[a] & B_JK row j) -- one popcount per set bit of A_IJ.
+// of popc(M[a] & B_JK row j) -- one popcount per set bit of A_IJ, with B's
+// tiles turned into row form once per call.
 //
 // Plain product sum (bmm{N}_sparse, bsrbmm/bsrbmm.cu:296-555):
 //   sum_{i,k} (A B)_ik = sum_j colsum_A(j) * rowsum_B(j),
@@ -64,6 +66,13 @@ struct TcArgs {
 // instead of by a group of bs lanes, one per tile row: R-MAT tiles hold 2-3
 // bits, so the group form left all but a few of its lanes idle.
 constexpr int32_t kLaneMatchBits = 16;
+
+// A mask tile (I,K) whose row I and column K both have bitmaps intersects
+// them by an AND of the two maps over the words both lists span when that
+// takes at most 1/kAndRatio of the passes of the key lookups (64 words or 64
+// keys a pass).  R-MAT s20: 5-13 % faster than the lookups alone with 1 or 2,
+// the key lookups' gathers being dearer than the AND's coalesced loads.
+constexpr int64_t kAndRatio = 2;
 
 // First index in [lo, hi) of sorted arr whose value is >= key.
 __device__ __forceinline__ int64_t lower_bound_i32(const int32_t* __restrict__ arr, int64_t lo, int64_t hi, int32_t key) {
@@ -206,33 +215,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
         const int32_t* lkeys = a_short ? a.bt_row : a.a_colind;
         // bs 32/64: one lane per tile row; mask row r stays in a register
         const uint64_t mrow = (BS >= 32 && lane < (kWave / BS) * BS) ? (uint64_t)At[tm * BS + lane % BS] : 0ull;
-        for (int64_t base = s0; base < s1; base += kWave) {
-          const int64_t q = base + lane;
-          int64_t tA = -1, tB = -1;
-          if (q < s1) {
-            const int32_t J = skeys[q];
-            if (use_map) {
-              const uint4 e = hm[J >> 6];  // word and rank in one load
-              const uint64_t wd = (uint64_t)e.x | ((uint64_t)e.y << 32);
-              if ((wd >> (J & 63)) & 1ull) {
-                tA = la0 + (int64_t)e.z + __popcll(wd & ((1ull << (J & 63)) - 1ull));
-                tB = a.bt_tile[q];
-              }
-            } else if (hk >= 0) {
-              const uint4 e = cm[J >> 6];
-              const uint64_t wd = (uint64_t)e.x | ((uint64_t)e.y << 32);
-              if ((wd >> (J & 63)) & 1ull) {
-                tA = q;
-                tB = a.bt_tile[lb0 + (int64_t)e.z + __popcll(wd & ((1ull << (J & 63)) - 1ull))];
-              }
-            } else {
-              const int64_t pos = lower_bound_i32(lkeys, l0, l1, J);
-              if (pos < l1 && lkeys[pos] == J) {
-                tA = a_short ? q : pos;
-                tB = a_short ? a.bt_tile[pos] : a.bt_tile[q];
-              }
-            }
-          }
+        // adds the matches (tA, tB) of the lanes that have one (tA >= 0); every
+        // lane of the wave calls it together
+        auto process = [&](int64_t tA, int64_t tB) {
           if constexpr (BS <= 16) {
             // each lane adds its own match
             if (tA >= 0) total += tc_hit<BS>(At + tA * BS, At + tm * BS, Rt + tB * BS, a.vec16 != 0);
@@ -278,6 +263,91 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
               }
             }
           }
+        };
+        // both rows have bitmaps: the common J are the set bits of the AND of
+        // the two maps over the words both lists span
+        const int32_t hK = (hs >= 0 && a.hcol) ? a.hcol[K] : -1;
+        if (hK >= 0 && la1 > la0 && lb1 > lb0) {
+          const int32_t jlo = max(a.a_colind[la0], a.bt_row[lb0]);
+          const int32_t jhi = min(a.a_colind[la1 - 1], a.bt_row[lb1 - 1]);
+          const int64_t len = min(la1 - la0, lb1 - lb0);
+          const int64_t wpass = jhi >= jlo ? ((jhi >> 6) - (jlo >> 6) + kWave) / kWave : 0;
+          if (wpass * kAndRatio <= (len + kWave - 1) / kWave) {
+            const uint4* __restrict__ km = a.hmap + (int64_t)hK * a.hwords;
+            for (int32_t w0 = jlo >> 6; jhi >= jlo && w0 <= (jhi >> 6); w0 += kWave) {
+              const int32_t w = w0 + lane;
+              uint64_t x = 0, wi = 0, wk = 0;
+              uint32_t ri = 0, rk = 0;
+              if (w <= (jhi >> 6)) {
+                const uint4 ei = hm[w], ek = km[w];
+                wi = (uint64_t)ei.x | ((uint64_t)ei.y << 32);
+                wk = (uint64_t)ek.x | ((uint64_t)ek.y << 32);
+                ri = ei.z;
+                rk = ek.z;
+                x = wi & wk;
+              }
+              // the chunk's matches spread over the lanes: match p is set bit
+              // k of the word of lane o (prefix sums of the words' popcounts)
+              const int cnt = __popcll(x);
+              const int incl = wave_incl_scan(cnt);
+              const int tot = __builtin_amdgcn_readlane(incl, kWave - 1);
+              for (int mb = 0; mb < tot; mb += kWave) {
+                const int p = mb + lane;
+                int o = 0;
+#pragma unroll
+                for (int step = kWave / 2; step >= 1; step >>= 1)
+                  if (__shfl(incl, o + step - 1) <= p) o += step;
+                o = o < kWave ? o : kWave - 1;
+                const int k = p - (__shfl(incl, o) - __shfl(cnt, o));
+                uint64_t xo = __shfl(x, o);
+                const uint64_t wio = __shfl(wi, o), wko = __shfl(wk, o);
+                const uint32_t rio = __shfl(ri, o), rko = __shfl(rk, o);
+                int64_t tA = -1, tB = -1;
+                if (p < tot) {
+                  // bit position of the k-th set bit of xo (k < popc(xo))
+                  int b = 0;
+#pragma unroll
+                  for (int step = 32; step >= 1; step >>= 1) {
+                    if (__popcll(xo & ((1ull << (b + step)) - 1ull)) <= k) b += step;  // b + step <= 63
+                  }
+                  const uint64_t below = (1ull << b) - 1ull;
+                  tA = la0 + (int64_t)rio + __popcll(wio & below);
+                  tB = a.bt_tile[lb0 + (int64_t)rko + __popcll(wko & below)];
+                }
+                process(tA, tB);
+              }
+            }
+            continue;
+          }
+        }
+        for (int64_t base = s0; base < s1; base += kWave) {
+          const int64_t q = base + lane;
+          int64_t tA = -1, tB = -1;
+          if (q < s1) {
+            const int32_t J = skeys[q];
+            if (use_map) {
+              const uint4 e = hm[J >> 6];  // word and rank in one load
+              const uint64_t wd = (uint64_t)e.x | ((uint64_t)e.y << 32);
+              if ((wd >> (J & 63)) & 1ull) {
+                tA = la0 + (int64_t)e.z + __popcll(wd & ((1ull << (J & 63)) - 1ull));
+                tB = a.bt_tile[q];
+              }
+            } else if (hk >= 0) {
+              const uint4 e = cm[J >> 6];
+              const uint64_t wd = (uint64_t)e.x | ((uint64_t)e.y << 32);
+              if ((wd >> (J & 63)) & 1ull) {
+                tA = q;
+                tB = a.bt_tile[lb0 + (int64_t)e.z + __popcll(wd & ((1ull << (J & 63)) - 1ull))];
+              }
+            } else {
+              const int64_t pos = lower_bound_i32(lkeys, l0, l1, J);
+              if (pos < l1 && lkeys[pos] == J) {
+                tA = a_short ? q : pos;
+                tB = a_short ? a.bt_tile[pos] : a.bt_tile[q];
+              }
+            }
+          }
+          process(tA, tB);
         }
       }
     }

@@ -231,6 +231,21 @@ def test_tc_lane_and_group_matches(gbo, po, bs):
     assert gbo.tc(king, bs) == 4 * 299 ** 2
 
 
+@pytest.mark.parametrize("bs", (8, 16, 32))
+def test_tc_bitmap_and(gbo, po, bs):
+    """Mask tiles whose row and column both have block-column bitmaps (rows
+    of more than 64 tiles) intersect them by an AND of the two maps.  A dense
+    random graph makes every row heavy; R-MAT makes only the hubs heavy."""
+    dense = gbo.gen_er(2400 if bs < 32 else 4800, 0.12, 9)
+    L = dense.tril(strict=True)
+    r = np.repeat(np.arange(L.nrows), np.diff(L.rowptr))
+    nb = (L.nrows + bs - 1) // bs
+    tiles_per_row = np.bincount(np.unique((r // bs) * nb + L.colind // bs) // nb, minlength=nb)
+    assert (tiles_per_row > 64).sum() >= nb // 3  # heavy rows: bitmaps on both sides
+    for g in (dense, gbo.gen_rmat(14, 16, seed=4)):
+        assert gbo.tc(g, bs) == po.tc(g.nrows, g.rowptr, g.colind)
+
+
 def test_tc_bs16_unaligned_tiles(gbo, po):
     """bs 16 reads A's tiles as 16-byte vectors when they are 16-byte aligned;
     tiles at an address that is not (a view 2 bytes into a buffer) take the
