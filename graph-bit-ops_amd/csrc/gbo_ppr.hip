@@ -11,13 +11,15 @@
 // checkpointing of the forward iterates).
 //
 // The product is a bit-BSR SpMM with a dense K-column right-hand side in
-// vertex-major layout X[n][K]: lane k of a wave owns column k, so every set
-// bit (r, c) of a tile costs one coalesced 4K-byte row read X[j][0..K).
-// At bs 8 (k_ppr_spmm8) the set bits are listed in LDS by all lanes and
-// read back 8 at a time, so 8 row reads are in flight per wave; X carries
-// the per-source scale (one elementwise pass).  The row reads are random
-// 256-byte accesses into an n x 256 B matrix: on MI355X such reads top out
-// near 2 TB/s, which bounds this product.  At bs 16/32 (k_ppr_spmm) tile
+// vertex-major layout X[n][K]: every set bit (r, c) of a tile costs one
+// 4K-byte row read X[j][0..K), taken by the wave together.  At bs 8 X
+// carries the per-source scale (one elementwise pass).  With whole blocks of
+// 64 columns (k_ppr_spmm8r) lane l holds tile row l / 8 and columns
+// 8 (l % 8) .. +7: the set bits are listed per tile row and the eight rows'
+// lists are walked in step, 32 bytes of a row read per lane.  Otherwise
+// (k_ppr_spmm8) lane k owns column k, and the set bits are listed in LDS
+// and read back 8 at a time.  Both add each row's terms in the same order,
+// so they give the same bits.  At bs 16/32 (k_ppr_spmm) tile
 // row words are fetched by lanes r < bs, the rows with bits are found by a
 // ballot and set bits are read one at a time.  Each row's partial sum lives
 // in register acc[r] (uniform index).  Waves take tile-balanced chunks;
@@ -250,6 +252,159 @@ __global__ __launch_bounds__(256) void k_ppr_spmm8(const PprArgs A) {
   }
 }
 
+// bs 8 SpMM for a block of 64 columns, lane = (tile row lane / 8, columns
+// 8 (lane % 8) .. +7).  A 64-tile block's set bits are listed per tile row
+// (eight LDS sublists; offsets from wave prefix sums of the rows' popcounts,
+// two rows to a 32-bit word as 16-bit fields), and the eight rows' sublists
+// are walked in step: each lane reads its row's next entry v and its 32
+// bytes of row v of X (two 16-byte loads), and adds them into eight FP64
+// registers of its own.  A row's entries keep the order of the flat list
+// (tile, then bit), so the sums are those of k_ppr_spmm8, with no
+// per-entry scalar work and no indexed register file access.
+constexpr int kPprRowCap = 128;  // sublist entries per tile row and pass
+// Entries per lane and step: four (eight 16-byte loads in flight per lane)
+// measured 8.2 ms per R-MAT s22 K = 64 iteration against 8.4 with two, at
+// 4 waves per SIMD (96-110 VGPRs); the flat-list kernel took 12.1.
+constexpr int kPprRowStep = 4;
+
+__device__ __forceinline__ uint32_t field16(uint32_t w, int hi) { return hi ? (w >> 16) : (w & 0xFFFFu); }
+
+__global__ __launch_bounds__(256) void k_ppr_spmm8r(const PprArgs A) {
+  constexpr int BS = 8, U = kPprRowStep, CAP = kPprRowCap;
+  typedef float f32x4 __attribute__((ext_vector_type(4)));
+  __shared__ uint32_t lists[256 / kWave][BS * CAP];
+  const int lane = lane_id();
+  const int row = lane >> 3, cg = lane & 7;
+  uint32_t* __restrict__ list = lists[threadIdx.x / kWave];
+  const int c = wave_index();
+  if (c >= A.nchunks) return;
+  const int64_t cb = chunk_start(A, c);
+  const int64_t ce = chunk_start(A, c + 1);
+  const bool last = (c == A.nchunks - 1);
+  const uint64_t* __restrict__ tw = (const uint64_t*)A.tiles;
+
+  for (int32_t R0 = A.plan[c]; R0 < A.nbr;) {
+    const int32_t Rl = R0 + lane;
+    int32_t rsl = 0, rel = 0;
+    if (Rl < A.nbr) {
+      rsl = A.rowptr[Rl];
+      rel = A.rowptr[Rl + 1];
+    }
+    const bool belong = (Rl < A.nbr) && (last || (int64_t)rsl < ce);
+    const uint64_t bmask = __ballot(belong);
+    uint64_t todo = bmask;  // empty rows are written with the epilogue too
+    while (todo) {
+      const int j = __builtin_ctzll(todo);
+      todo &= todo - 1;
+      const int64_t R = R0 + j;
+      const int64_t rs = __builtin_amdgcn_readlane(rsl, j), re = __builtin_amdgcn_readlane(rel, j);
+      const int64_t s = rs > cb ? rs : cb;
+      const int64_t te = re < ce ? re : ce;
+      const bool full = (rs >= cb) && (re <= ce);
+      double acc[8];
+#pragma unroll
+      for (int m = 0; m < 8; ++m) acc[m] = 0.0;
+      for (int64_t b = s; b < te; b += kWave) {
+        const int64_t nt = min((int64_t)kWave, te - b);
+        const uint64_t word = lane < nt ? tw[b + lane] : 0ull;
+        const uint32_t vb = lane < nt ? (uint32_t)A.colind[b + lane] * BS + (BS - 1) : 0u;
+        int maxlen = 0, mylen = 0;
+        {
+          // row totals of the block (sublist lengths)
+          uint32_t tot[4];
+#pragma unroll
+          for (int h = 0; h < 4; ++h) {
+            const uint32_t qh = (uint32_t)__popc((uint32_t)(word >> (16 * h)) & 0xFFu) |
+                                ((uint32_t)__popc((uint32_t)(word >> (16 * h + 8)) & 0xFFu) << 16);
+            tot[h] = __builtin_amdgcn_readlane((uint32_t)wave_incl_scan((int)qh), kWave - 1);
+          }
+#pragma unroll
+          for (int r = 0; r < BS; ++r) {
+            const int l = (int)field16(tot[r >> 1], r & 1);
+            maxlen = l > maxlen ? l : maxlen;
+            mylen = (row == r) ? l : mylen;
+          }
+        }
+        for (int pass = 0; pass < maxlen; pass += CAP) {
+          // this lane's bits of every row whose sublist index is in the pass
+          // (offsets recomputed per pass, so that they are not live across
+          // the gathers)
+#pragma unroll
+          for (int r = 0; r < BS; ++r) {
+            uint32_t byte = (uint32_t)(word >> (8 * r)) & 0xFFu;
+            const uint32_t qr = (uint32_t)__popc(byte);
+            int k = wave_incl_scan((int)qr) - (int)qr - pass;
+            while (byte) {
+              const int bit = __builtin_ctz(byte);  // column 7 - bit
+              byte &= byte - 1;
+              if (k >= 0 && k < CAP) list[r * CAP + k] = vb - (uint32_t)bit;
+              ++k;
+            }
+          }
+          __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+          const int lr = min(max(mylen - pass, 0), CAP);
+          const int kmax = min(maxlen - pass, CAP);
+          for (int k0 = 0; k0 < kmax; k0 += U) {
+            f32x4 xa[U], xb[U];
+            bool ok[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+              ok[u] = k0 + u < lr;
+              const uint32_t v = ok[u] ? list[row * CAP + k0 + u] : 0u;  // row 0 stands in
+              const f32x4* px = (const f32x4*)(A.X + (int64_t)v * A.ld + cg * 8);
+              xa[u] = px[0];
+              xb[u] = px[1];
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+              if (ok[u]) {
+                acc[0] += (double)xa[u].x;
+                acc[1] += (double)xa[u].y;
+                acc[2] += (double)xa[u].z;
+                acc[3] += (double)xa[u].w;
+                acc[4] += (double)xb[u].x;
+                acc[5] += (double)xb[u].y;
+                acc[6] += (double)xb[u].z;
+                acc[7] += (double)xb[u].w;
+              }
+            }
+          }
+          __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        }
+      }
+      const int64_t i = R * BS + row;
+      if (i < A.nrows) {
+        if (full) {
+          const double rsc = A.rowscale ? (double)A.rowscale[i] : 1.0;
+          f32x4 za = {0.f, 0.f, 0.f, 0.f}, zb = za;
+          if (A.Z) {
+            const f32x4* pz = (const f32x4*)(A.Z + i * A.ld + cg * 8);
+            za = pz[0];
+            zb = pz[1];
+          }
+          const float zz[8] = {za.x, za.y, za.z, za.w, zb.x, zb.y, zb.z, zb.w};
+          float y[8];
+#pragma unroll
+          for (int m = 0; m < 8; ++m) {
+            double t = (double)A.a * rsc * acc[m];
+            if (A.Z) t += (double)A.b * (double)zz[m];
+            y[m] = (float)t;
+          }
+          f32x4* py = (f32x4*)(A.Y + i * A.ld + cg * 8);
+          py[0] = f32x4{y[0], y[1], y[2], y[3]};
+          py[1] = f32x4{y[4], y[5], y[6], y[7]};
+        } else {
+          float* pp = A.partial + (((int64_t)c * 2 + ((rs < cb) ? 0 : 1)) * BS + row) * kWave + cg * 8;
+#pragma unroll
+          for (int m = 0; m < 8; ++m) pp[m] = (float)acc[m];
+        }
+      }
+    }
+    if (__popcll(bmask) < kWave) break;
+    R0 += kWave;
+  }
+}
+
 // Xs[i][k] = X[i][k] * scale[i] over K columns of a row stride ld.
 __global__ __launch_bounds__(256) void k_row_scale(const float* __restrict__ X, const float* __restrict__ scale,
                                                    float* __restrict__ Xs, int32_t n, int32_t K, int32_t ld) {
@@ -396,6 +551,17 @@ __global__ void k_plan_ppr(const int32_t* __restrict__ rowptr, int32_t nbr, int3
   plan[c] = R;
 }
 
+// k_row_scale for K % 4 == 0 and 16-byte aligned X and Xs (ld == K):
+// four columns per thread, n * K < 2^32.
+__global__ __launch_bounds__(256) void k_row_scale4(const float* __restrict__ X, const float* __restrict__ scale,
+                                                    float* __restrict__ Xs, uint32_t n4, uint32_t K4) {
+  typedef float f32x4 __attribute__((ext_vector_type(4)));
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n4) return;
+  const uint32_t i = t / K4;
+  ((f32x4*)Xs)[t] = ((const f32x4*)X)[t] * scale[i];
+}
+
 static int make_ppr_plan(const gbo_bsr* M, PprPlan* P, hipStream_t st) {
   P->chunk = 1024;
   P->nchunks = (int32_t)std::max<int64_t>(1, (M->nblocks + P->chunk - 1) / P->chunk);
@@ -421,12 +587,20 @@ static int spmm(const gbo_bsr* M, PprPlan& P, int32_t K, const float* X, const f
       P.xs_len = len;
     }
     if (len > 0) {
-      k_row_scale<<<(unsigned)((len + 255) / 256), 256, 0, st>>>(X, colscale, P.xs, M->ncols, K, K);
+      if (K % 4 == 0 && len < ((int64_t)1 << 32) && ((((uintptr_t)X) | ((uintptr_t)P.xs)) & 15) == 0) {
+        const uint32_t n4 = (uint32_t)(len / 4);
+        k_row_scale4<<<(n4 + 255) / 256, 256, 0, st>>>(X, colscale, P.xs, n4, (uint32_t)(K / 4));
+      } else {
+        k_row_scale<<<(unsigned)((len + 255) / 256), 256, 0, st>>>(X, colscale, P.xs, M->ncols, K, K);
+      }
       GBO_LAUNCH_CHECK();
     }
     X = P.xs;
     colscale = nullptr;
   }
+  // bs 8 with whole 64-column blocks and 16-byte aligned rows: the per-row
+  // sublist kernel (the same sums as the flat-list one, fewer instructions)
+  const bool rows8 = M->bs == 8 && K % kWave == 0 && ((((uintptr_t)X) | ((uintptr_t)Y) | ((uintptr_t)Z)) & 15) == 0;
   for (int32_t k0 = 0; k0 < K; k0 += kWave) {
     PprArgs A{};
     A.rowptr = M->rowptr;
@@ -450,7 +624,11 @@ static int spmm(const gbo_bsr* M, PprPlan& P, int32_t K, const float* X, const f
     A.partial = P.partial;
     const unsigned blocks = (unsigned)(((int64_t)P.nchunks + 3) / 4);
     switch (M->bs) {
-      case 8: k_ppr_spmm8<<<blocks, 256, 0, st>>>(A); break;
+      case 8: {
+        if (rows8) k_ppr_spmm8r<<<blocks, 256, 0, st>>>(A);
+        else k_ppr_spmm8<<<blocks, 256, 0, st>>>(A);
+        break;
+      }
       case 16: k_ppr_spmm<16><<<blocks, 256, 0, st>>>(A); break;
       case 32: k_ppr_spmm<32><<<blocks, 256, 0, st>>>(A); break;
       default: return set_error(GBO_EUNSUPPORTED, "PPR supports block sizes 8, 16, 32");

@@ -333,6 +333,41 @@ def _ppr_case(gbo, seed, n_seeds=20, K=8):
     return g, S, W
 
 
+def test_ppr_bs8_kernels_give_the_same_bits(gbo):
+    """At bs 8 with K = 64 and 16-byte aligned rows the SpMM lists the set bits
+    per tile row (k_ppr_spmm8r); otherwise it uses one flat list
+    (k_ppr_spmm8).  Both add each row's terms in the same order, so a
+    fixed number of forward and adjoint iterations gives the same bits when
+    S (forward) or W (adjoint) is moved 4 bytes off alignment."""
+    g, S, W = _ppr_case(gbo, 7, n_seeds=30, K=64)
+    n, K = g.nrows, 64
+    A = gbo.BsrMatrix.from_csr(g, 8)
+    d_od = gbo.DeviceBuffer.from_numpy(g.outdeg().astype(np.int32))
+    S32, W32 = S.astype(np.float32), W.astype(np.float32)
+    nb = S32.nbytes
+
+    def placed(arr, off):
+        buf = gbo.DeviceBuffer(nb + 16)
+        gbo._check(gbo.lib().gbo_memcpy_h2d(buf.ptr + off, arr.ctypes.data, nb))
+        return buf, buf.ptr + off
+
+    out = {}
+    for off in (0, 4):
+        bs_, ps = placed(S32, off)
+        bw_, pw = placed(W32, off)
+        d_p, d_g = gbo.DeviceBuffer(nb), gbo.DeviceBuffer(nb)
+        it, _ = gbo.ppr_forward_ptr(A, d_od.ptr, K, 0.85, ps, d_p.ptr, 0.0, 9, strict=False)
+        ga, ita = gbo.ppr_adjoint_ptr(A, A, d_od.ptr, K, 0.85, ps, d_p.ptr, pw, d_g.ptr, 0.0, 9, strict=False)
+        assert it == 9 and ita == 9
+        out[off] = (d_p.to_numpy(np.float32, n * K), d_g.to_numpy(np.float32, n * K), ga)
+        for b in (bs_, bw_, d_p, d_g):
+            b.free()
+    assert np.array_equal(out[0][0].view(np.uint32), out[4][0].view(np.uint32))
+    assert np.array_equal(out[0][1].view(np.uint32), out[4][1].view(np.uint32))
+    assert out[0][2] == out[4][2]
+    assert np.abs(out[0][0]).sum() > 0 and np.abs(out[0][1]).sum() > 0
+
+
 @pytest.mark.parametrize("bs", (8, 16, 32))
 def test_ppr_forward_adjoint(gbo, bs):
     """GPU PPR forward (FP32) within 1e-6 L1 per column of the FP64 fixed
