@@ -113,6 +113,39 @@ enum { PACK_COUNT = 0, PACK_ROW = 1, PACK_COL = 2, PACK_VALUES = 3 };
 // Error flags raised by the packer (device int, OR-ed).
 enum { PACKERR_COLRANGE = 1, PACKERR_UNSORTED = 2 };
 
+// A block row with more than kPieceNnz nonzeros is packed in pieces: ranges
+// of block columns [lo, hi), one group each, cut at evenly spaced entries of
+// the block row's longest CSR row.  A single group's merge over the hub
+// block row of R-MAT s22 (129,447 tiles at bs 8, one dependent step per
+// tile) had set the time of each whole packing pass (~75 ms).
+constexpr int32_t kPieceNnz = 2048;
+
+// The pieces: piece p is block row row[p] over block columns [lo[p], hi[p])
+// (lo = INT_MIN: from the row's start, hi = INT_MAX: to its end).  Without a
+// table (row == nullptr) piece p is block row p, whole.
+struct PackPieces {
+  const int32_t* row = nullptr;
+  const int32_t* lo = nullptr;
+  const int32_t* hi = nullptr;
+  const int64_t* base = nullptr;  // fill: first tile of each piece
+  int32_t* cnt = nullptr;         // count: tiles of each piece
+  int32_t n = 0;
+};
+
+// First index q in [q0, q1) with ci[q] / BS >= J (q1 if none).  On rows that
+// are not sorted the result is still monotone in J, so the pieces of a row
+// cover it exactly and the count pass sees every adjacent pair once.
+template <int BS>
+__device__ __forceinline__ int32_t first_blockcol_at_least(const int32_t* __restrict__ ci, int32_t q0, int32_t q1,
+                                                           int32_t J) {
+  while (q0 < q1) {
+    const int32_t mid = q0 + (q1 - q0) / 2;
+    if (ci[mid] / BS < J) q0 = mid + 1;
+    else q1 = mid;
+  }
+  return q0;
+}
+
 template <int BS, int MODE>
 __global__ __launch_bounds__(256) void k_pack(const int32_t* __restrict__ rp,
                                               const int32_t* __restrict__ ci, int32_t nrows,
@@ -122,20 +155,32 @@ __global__ __launch_bounds__(256) void k_pack(const int32_t* __restrict__ rp,
                                               word_t<BS>* __restrict__ out_tiles,
                                               const uint32_t* __restrict__ val,
                                               const int64_t* __restrict__ woff,
-                                              uint32_t* __restrict__ tval, int* __restrict__ err) {
+                                              uint32_t* __restrict__ tval, int* __restrict__ err,
+                                              const PackPieces pc) {
   using W = word_t<BS>;
   constexpr int G = kWave / BS;
   const int lane = lane_id();
   const int g = lane / BS, r = lane % BS;
   const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
-  const int64_t I = wave * G + g;
-  if (wave * G >= nbr) return;  // wave-uniform
-  const bool blk_ok = I < nbr;
+  const int64_t npc = pc.row ? (int64_t)pc.n : (int64_t)nbr;
+  const int64_t pi = wave * G + g;  // piece of this group
+  if (wave * G >= npc) return;      // wave-uniform
+  const bool blk_ok = pi < npc;
+  const int64_t I = pc.row ? (blk_ok ? (int64_t)pc.row[pi] : 0) : pi;
   const int64_t row = I * BS + r;
   int32_t p = 0, pe = 0;
   if (blk_ok && row < nrows) {
-    p = rp[row];
-    pe = rp[row + 1];
+    const int32_t r0 = rp[row], r1 = rp[row + 1];
+    p = r0;
+    pe = r1;
+    if (pc.row) {
+      const int32_t lo = pc.lo[pi], hi = pc.hi[pi];
+      if (lo != INT_MIN) p = first_blockcol_at_least<BS>(ci, r0, r1, lo);
+      if (hi != INT_MAX) pe = first_blockcol_at_least<BS>(ci, r0, r1, hi);
+      if (pe < p) pe = p;  // only on rows that are not sorted (flagged below)
+      // the pair across the piece's start (pairs inside it are checked by the merge)
+      if (MODE == PACK_COUNT && p > r0 && p < r1 && ci[p - 1] / BS > ci[p] / BS) atomicOr(err, PACKERR_UNSORTED);
+    }
   }
   auto blockcol = [&](int32_t q) -> int32_t {
     const int32_t c = ci[q];
@@ -147,7 +192,7 @@ __global__ __launch_bounds__(256) void k_pack(const int32_t* __restrict__ rp,
   };
   int32_t cur = (p < pe) ? blockcol(p) : INT_MAX;
   int64_t base = 0;
-  if (MODE != PACK_COUNT && blk_ok) base = bsr_rp[I];
+  if (MODE != PACK_COUNT && blk_ok) base = pc.base ? pc.base[pi] : bsr_rp[I];
   int32_t k = 0;
   while (true) {
     const int32_t m = group_min<BS>(cur);
@@ -204,8 +249,55 @@ __global__ __launch_bounds__(256) void k_pack(const int32_t* __restrict__ rp,
     if (m != INT_MAX) ++k;
   }
   if constexpr (MODE == PACK_COUNT) {
-    if (blk_ok && r == 0) bsr_rp[I] = k;
+    if (blk_ok && r == 0) {
+      if (pc.cnt) pc.cnt[pi] = k;
+      else bsr_rp[I] = k;
+    }
   }
+}
+
+// Pieces per block row.
+__global__ __launch_bounds__(256) void k_piece_n(const int32_t* __restrict__ rp, int32_t nrows, int32_t nbr, int bs,
+                                                 int32_t* __restrict__ pn) {
+  const int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (I >= nbr) return;
+  const int64_t r0 = I * bs, r1 = min(r0 + bs, (int64_t)nrows);
+  const int64_t nnz = (int64_t)rp[r1] - rp[r0];
+  pn[I] = (int32_t)max((int64_t)1, (nnz + kPieceNnz - 1) / kPieceNnz);
+}
+
+// Piece table: block row I's pieces from poff[I], cut at evenly spaced
+// entries of its longest CSR row.
+__global__ __launch_bounds__(256) void k_piece_fill(const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
+                                                    int32_t nrows, int32_t nbr, int bs,
+                                                    const int32_t* __restrict__ poff, int32_t* __restrict__ prow,
+                                                    int32_t* __restrict__ plo, int32_t* __restrict__ phi) {
+  const int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (I >= nbr) return;
+  const int32_t o = poff[I], P = poff[I + 1] - poff[I];
+  int32_t best = 0, len = -1;
+  const int64_t r0 = I * bs, r1 = min(r0 + bs, (int64_t)nrows);
+  if (P > 1) {
+    for (int64_t rr = r0; rr < r1; ++rr) {
+      const int32_t l = rp[rr + 1] - rp[rr];
+      if (l > len) {
+        len = l;
+        best = rp[rr];
+      }
+    }
+  }
+  for (int32_t j = 0; j < P; ++j) {
+    prow[o + j] = (int32_t)I;
+    plo[o + j] = j == 0 ? INT_MIN : ci[best + (int32_t)((int64_t)j * len / P)] / bs;
+    phi[o + j] = j == P - 1 ? INT_MAX : ci[best + (int32_t)((int64_t)(j + 1) * len / P)] / bs;
+  }
+}
+
+// rowptr[I] = first tile of block row I's first piece (rowptr[nbr] = total).
+__global__ __launch_bounds__(256) void k_piece_rowptr(const int32_t* __restrict__ poff, const int64_t* __restrict__ toff,
+                                                      int32_t nbr, int32_t* __restrict__ rowptr) {
+  const int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (I <= nbr) rowptr[I] = (int32_t)toff[poff[I]];
 }
 
 template <int BS>
@@ -270,13 +362,14 @@ static bool bs_ok(int bs) { return bs == 8 || bs == 16 || bs == 32 || bs == 64; 
 template <int BS, int MODE>
 static int launch_pack(const int32_t* rp, const int32_t* ci, int32_t nrows, int32_t ncols, int32_t nbr,
                        int32_t* bsr_rp, int32_t* out_col, void* out_tiles, const uint32_t* val,
-                       const int64_t* woff, uint32_t* tval, int* err, hipStream_t st) {
+                       const int64_t* woff, uint32_t* tval, int* err, const PackPieces& pc, hipStream_t st) {
   constexpr int G = kWave / BS;
-  const int64_t waves = ((int64_t)nbr + G - 1) / G;
+  const int64_t units = pc.row ? (int64_t)pc.n : (int64_t)nbr;
+  const int64_t waves = (units + G - 1) / G;
   const int64_t blocks = (waves + 3) / 4;
   if (blocks == 0) return GBO_OK;
   k_pack<BS, MODE><<<(unsigned)blocks, 256, 0, st>>>(rp, ci, nrows, ncols, nbr, bsr_rp, out_col,
-                                                     (word_t<BS>*)out_tiles, val, woff, tval, err);
+                                                     (word_t<BS>*)out_tiles, val, woff, tval, err, pc);
   GBO_LAUNCH_CHECK();
   return GBO_OK;
 }
@@ -285,14 +378,85 @@ template <int MODE>
 static int dispatch_pack(int bs, const int32_t* rp, const int32_t* ci, int32_t nrows, int32_t ncols,
                          int32_t nbr, int32_t* bsr_rp, int32_t* out_col, void* out_tiles,
                          const uint32_t* val, const int64_t* woff, uint32_t* tval, int* err,
-                         hipStream_t st) {
+                         const PackPieces& pc, hipStream_t st) {
   switch (bs) {
-    case 8: return launch_pack<8, MODE>(rp, ci, nrows, ncols, nbr, bsr_rp, out_col, out_tiles, val, woff, tval, err, st);
-    case 16: return launch_pack<16, MODE>(rp, ci, nrows, ncols, nbr, bsr_rp, out_col, out_tiles, val, woff, tval, err, st);
-    case 32: return launch_pack<32, MODE>(rp, ci, nrows, ncols, nbr, bsr_rp, out_col, out_tiles, val, woff, tval, err, st);
-    case 64: return launch_pack<64, MODE>(rp, ci, nrows, ncols, nbr, bsr_rp, out_col, out_tiles, val, woff, tval, err, st);
+    case 8: return launch_pack<8, MODE>(rp, ci, nrows, ncols, nbr, bsr_rp, out_col, out_tiles, val, woff, tval, err, pc, st);
+    case 16: return launch_pack<16, MODE>(rp, ci, nrows, ncols, nbr, bsr_rp, out_col, out_tiles, val, woff, tval, err, pc, st);
+    case 32: return launch_pack<32, MODE>(rp, ci, nrows, ncols, nbr, bsr_rp, out_col, out_tiles, val, woff, tval, err, pc, st);
+    case 64: return launch_pack<64, MODE>(rp, ci, nrows, ncols, nbr, bsr_rp, out_col, out_tiles, val, woff, tval, err, pc, st);
   }
   return set_error(GBO_EUNSUPPORTED, "block size %d not supported (8, 16, 32, 64)", bs);
+}
+
+static int check_err_flag(int* d_err, hipStream_t st, const char* who) {
+  int h = 0;
+  GBO_HIP_CHECK(hipMemcpyAsync(&h, d_err, sizeof(int), hipMemcpyDeviceToHost, st));
+  GBO_HIP_CHECK(hipStreamSynchronize(st));
+  if (h & PACKERR_COLRANGE) return set_error(GBO_EINVAL, "%s: column index out of range", who);
+  if (h & PACKERR_UNSORTED) return set_error(GBO_EINVAL, "%s: CSR rows must be sorted ascending", who);
+  return GBO_OK;
+}
+
+// The pieces of every block row, their tile counts (a count pass that also
+// checks the rows: columns in range, block columns ascending) and the first
+// tile of each (toff, npieces + 1 entries); *total = tiles.  Buffers are
+// owned by `mem`; poff (nbr + 1) is each block row's first piece.
+struct PieceSet {
+  PackPieces pc;
+  int32_t* poff = nullptr;
+  int64_t* toff = nullptr;
+  int64_t total = 0;
+};
+
+static int make_pieces(int bs, const int32_t* d_rp, const int32_t* d_ci, int32_t nrows, int32_t ncols, int32_t nbr,
+                       AsyncAllocs& mem, PieceSet* ps, const char* who, hipStream_t st) {
+  int rc;
+  int32_t *poff = nullptr, *prow = nullptr, *plo = nullptr, *phi = nullptr, *pcnt = nullptr;
+  int64_t* toff = nullptr;
+  int* d_err = nullptr;
+  if ((rc = mem.alloc(&poff, sizeof(int32_t) * ((size_t)nbr + 1), who)) || (rc = mem.alloc(&d_err, sizeof(int), who)))
+    return rc;
+  GBO_HIP_CHECK(hipMemsetAsync(d_err, 0, sizeof(int), st));
+  if (nbr > 0) {
+    k_piece_n<<<(unsigned)(((int64_t)nbr + 255) / 256), 256, 0, st>>>(d_rp, nrows, nbr, bs, poff);
+    GBO_LAUNCH_CHECK();
+  }
+  if ((rc = scan_exclusive<int32_t, int32_t>(poff, poff, nbr, st))) return rc;
+  int32_t np = 0;
+  GBO_HIP_CHECK(hipMemcpyAsync(&np, poff + nbr, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  GBO_HIP_CHECK(hipStreamSynchronize(st));
+  if ((rc = mem.alloc(&prow, sizeof(int32_t) * std::max<int64_t>(np, 1), who)) ||
+      (rc = mem.alloc(&plo, sizeof(int32_t) * std::max<int64_t>(np, 1), who)) ||
+      (rc = mem.alloc(&phi, sizeof(int32_t) * std::max<int64_t>(np, 1), who)) ||
+      (rc = mem.alloc(&pcnt, sizeof(int32_t) * std::max<int64_t>(np, 1), who)) ||
+      (rc = mem.alloc(&toff, sizeof(int64_t) * ((size_t)np + 1), who)))
+    return rc;
+  if (nbr > 0) {
+    k_piece_fill<<<(unsigned)(((int64_t)nbr + 255) / 256), 256, 0, st>>>(d_rp, d_ci, nrows, nbr, bs, poff, prow, plo,
+                                                                         phi);
+    GBO_LAUNCH_CHECK();
+  }
+  PackPieces pc;
+  pc.row = prow;
+  pc.lo = plo;
+  pc.hi = phi;
+  pc.cnt = pcnt;
+  pc.n = np;
+  if ((rc = dispatch_pack<PACK_COUNT>(bs, d_rp, d_ci, nrows, ncols, nbr, nullptr, nullptr, nullptr, nullptr,
+                                      nullptr, nullptr, d_err, pc, st)))
+    return rc;
+  if ((rc = check_err_flag(d_err, st, who))) return rc;
+  if ((rc = scan_exclusive<int32_t, int64_t>(pcnt, toff, np, st))) return rc;
+  GBO_HIP_CHECK(hipMemcpyAsync(&ps->total, toff + np, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  GBO_HIP_CHECK(hipStreamSynchronize(st));
+  if (ps->total >= ((int64_t)1 << 31))
+    return set_error(GBO_ERANGE, "tile count %lld exceeds int32", (long long)ps->total);
+  pc.cnt = nullptr;
+  pc.base = toff;
+  ps->pc = pc;
+  ps->poff = poff;
+  ps->toff = toff;
+  return GBO_OK;
 }
 
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -408,62 +572,59 @@ int validate_bsr(const gbo_bsr* A, int want_dir, const char* who) {
 
 using namespace gbo;
 
-static int check_err_flag(int* d_err, hipStream_t st, const char* who) {
-  int h = 0;
-  GBO_HIP_CHECK(hipMemcpyAsync(&h, d_err, sizeof(int), hipMemcpyDeviceToHost, st));
-  GBO_HIP_CHECK(hipStreamSynchronize(st));
-  if (h & PACKERR_COLRANGE) return set_error(GBO_EINVAL, "%s: column index out of range", who);
-  if (h & PACKERR_UNSORTED) return set_error(GBO_EINVAL, "%s: CSR rows must be sorted ascending", who);
-  return GBO_OK;
-}
 
 extern "C" {
 
 int gbo_csr2bsr_nnzb_dev(int bs, const int32_t* d_rp, const int32_t* d_ci, int32_t nrows, int32_t ncols,
                          int32_t* d_bsrRowPtr, int64_t* nblocks, gbo_stream_t stream) {
+  GBO_TRY_BEGIN
   if (!bs_ok(bs)) return set_error(GBO_EUNSUPPORTED, "block size %d not supported (8, 16, 32, 64)", bs);
   if (nrows < 0 || ncols < 0 || !d_rp || !d_bsrRowPtr || !nblocks)
     return set_error(GBO_EINVAL, "gbo_csr2bsr_nnzb_dev: bad argument");
   hipStream_t st = stream;
   const int32_t nbr = (nrows + bs - 1) / bs;
-  int* d_err = nullptr;
-  GBO_HIP_CHECK(hipMallocAsync((void**)&d_err, sizeof(int), st));
-  GBO_HIP_CHECK(hipMemsetAsync(d_err, 0, sizeof(int), st));
-  int rc = dispatch_pack<PACK_COUNT>(bs, d_rp, d_ci, nrows, ncols, nbr, d_bsrRowPtr, nullptr, nullptr,
-                                     nullptr, nullptr, nullptr, d_err, st);
-  if (rc == GBO_OK) rc = check_err_flag(d_err, st, "gbo_csr2bsr_nnzb_dev");
-  (void)hipFreeAsync(d_err, st);
+  AsyncAllocs mem(st);
+  PieceSet ps;
+  int rc = make_pieces(bs, d_rp, d_ci, nrows, ncols, nbr, mem, &ps, "gbo_csr2bsr_nnzb_dev", st);
   if (rc != GBO_OK) return rc;
-  // counts -> exclusive scan (in place); total must fit int32.
-  int64_t* tmp = nullptr;
-  GBO_HIP_CHECK(hipMallocAsync((void**)&tmp, sizeof(int64_t) * ((size_t)nbr + 1), st));
-  rc = scan_exclusive<int32_t, int64_t>(d_bsrRowPtr, tmp, nbr, st);
-  int64_t total = 0;
-  if (rc == GBO_OK) {
-    GBO_HIP_CHECK(hipMemcpyAsync(&total, tmp + nbr, sizeof(int64_t), hipMemcpyDeviceToHost, st));
-    GBO_HIP_CHECK(hipStreamSynchronize(st));
-    if (total >= ((int64_t)1 << 31)) rc = set_error(GBO_ERANGE, "tile count %lld exceeds int32", (long long)total);
-  }
-  if (rc == GBO_OK) rc = scan_exclusive<int32_t, int32_t>(d_bsrRowPtr, d_bsrRowPtr, nbr, st);
-  (void)hipFreeAsync(tmp, st);
-  if (rc != GBO_OK) return rc;
+  k_piece_rowptr<<<(unsigned)(((int64_t)nbr + 1 + 255) / 256), 256, 0, st>>>(ps.poff, ps.toff, nbr, d_bsrRowPtr);
+  GBO_LAUNCH_CHECK();
   GBO_HIP_CHECK(hipStreamSynchronize(st));
-  *nblocks = total;
+  *nblocks = ps.total;
   return GBO_OK;
+  GBO_TRY_END
 }
 
 int gbo_csr2bsr_fill_dev(int bs, int dir, const int32_t* d_rp, const int32_t* d_ci, int32_t nrows,
                          int32_t ncols, const int32_t* d_bsrRowPtr, int32_t* d_bsrColInd, void* d_tiles,
                          gbo_stream_t stream) {
+  GBO_TRY_BEGIN
   if (!bs_ok(bs)) return set_error(GBO_EUNSUPPORTED, "block size %d not supported (8, 16, 32, 64)", bs);
   if (dir != 0 && dir != 1) return set_error(GBO_EINVAL, "dir must be 0 (row) or 1 (col)");
   if (nrows < 0 || ncols < 0 || !d_rp || !d_bsrRowPtr) return set_error(GBO_EINVAL, "gbo_csr2bsr_fill_dev: bad argument");
+  hipStream_t st = stream;
   const int32_t nbr = (nrows + bs - 1) / bs;
+  // the tile offsets come from the pieces' counts (the same as d_bsrRowPtr
+  // at block-row starts); the rows are checked again on the way
+  AsyncAllocs mem(st);
+  PieceSet ps;
+  int rc = make_pieces(bs, d_rp, d_ci, nrows, ncols, nbr, mem, &ps, "gbo_csr2bsr_fill_dev", st);
+  if (rc != GBO_OK) return rc;
+  if (ps.total > 0 && (!d_bsrColInd || !d_tiles)) return set_error(GBO_EINVAL, "gbo_csr2bsr_fill_dev: null output");
+  int32_t given = 0;  // the caller sized colind and tiles by rowptr[nbr]
+  GBO_HIP_CHECK(hipMemcpyAsync(&given, d_bsrRowPtr + nbr, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  GBO_HIP_CHECK(hipStreamSynchronize(st));
+  if (given != ps.total)
+    return set_error(GBO_EINVAL, "gbo_csr2bsr_fill_dev: rowptr[nblockrows] = %d, the CSR has %lld tiles", given,
+                     (long long)ps.total);
   if (dir == 0)
-    return dispatch_pack<PACK_ROW>(bs, d_rp, d_ci, nrows, ncols, nbr, const_cast<int32_t*>(d_bsrRowPtr),
-                                   d_bsrColInd, d_tiles, nullptr, nullptr, nullptr, nullptr, stream);
-  return dispatch_pack<PACK_COL>(bs, d_rp, d_ci, nrows, ncols, nbr, const_cast<int32_t*>(d_bsrRowPtr),
-                                 d_bsrColInd, d_tiles, nullptr, nullptr, nullptr, nullptr, stream);
+    rc = dispatch_pack<PACK_ROW>(bs, d_rp, d_ci, nrows, ncols, nbr, const_cast<int32_t*>(d_bsrRowPtr), d_bsrColInd,
+                                 d_tiles, nullptr, nullptr, nullptr, nullptr, ps.pc, st);
+  else
+    rc = dispatch_pack<PACK_COL>(bs, d_rp, d_ci, nrows, ncols, nbr, const_cast<int32_t*>(d_bsrRowPtr), d_bsrColInd,
+                                 d_tiles, nullptr, nullptr, nullptr, nullptr, ps.pc, st);
+  return rc;
+  GBO_TRY_END
 }
 
 int gbo_bsr_create_dev(int bs, int dir, const int32_t* d_rp, const int32_t* d_ci, int32_t nrows,
@@ -642,8 +803,14 @@ int gbo_csr2bsr_values_u32(const gbo_bsr* A, const int32_t* d_rp, const int32_t*
   rc = scan_exclusive<int64_t, int64_t>(cnt, d_woff, T, st);
   (void)hipFreeAsync(cnt, st);
   if (rc) return rc;
+  AsyncAllocs mem(st);
+  PieceSet ps;
+  if ((rc = make_pieces(A->bs, d_rp, d_ci, A->nrows, A->ncols, A->nblockrows, mem, &ps, "gbo_csr2bsr_values_u32", st)))
+    return rc;
+  if (ps.total != T) return set_error(GBO_EINVAL, "gbo_csr2bsr_values_u32: the CSR has %lld tiles, the matrix %lld",
+                                      (long long)ps.total, (long long)T);
   return dispatch_pack<PACK_VALUES>(A->bs, d_rp, d_ci, A->nrows, A->ncols, A->nblockrows, A->rowptr, nullptr,
-                                    nullptr, d_val, d_woff, d_tval, nullptr, st);
+                                    nullptr, d_val, d_woff, d_tval, nullptr, ps.pc, st);
 }
 
 int gbo_pack_vec(int bs, const float* d_x, void* d_xbits, int32_t n, gbo_stream_t stream) {
