@@ -9,8 +9,10 @@
 // block row, the group takes the minimum pending block column (a k-way merge
 // of the bs sorted rows), and every lane ORs its entries of that block
 // column into its own row word -- which is directly word r of the tile in
-// the reference layout (MSB = column 0).  Two passes: count tiles per block
-// row, scan, then fill colind/tiles.  No dense staging, no host round trip.
+// the reference layout (MSB = column 0).  A block row of more than 2048
+// nonzeros is split into pieces (block-column ranges) merged by groups of
+// their own.  Two passes: count tiles per piece, scan, then fill
+// colind/tiles.  No dense staging.
 #include <algorithm>
 #include <cstring>
 #include <vector>
