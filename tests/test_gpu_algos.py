@@ -308,11 +308,12 @@ def test_bsrbmm_operands(gbo, po, bs):
 @pytest.mark.parametrize("bs", BS)
 def test_bfs_multisource(gbo, po, graph_cases, bs):
     """Batched BFS == per-source BFS (oracle queue BFS), bit-exact, for 1, 64,
-    100 and 256 sources including duplicates and isolated vertices."""
+    100, 150 and 256 sources (one to four 64-bit frontier words per vertex)
+    including duplicates and isolated vertices."""
     rng = np.random.default_rng(bs)
     for name, g in graph_cases:
         A = gbo.BsrMatrix.from_csr(g, bs)
-        for k in (1, 64, 100, 256):
+        for k in (1, 64, 100, 150, 256):
             src = rng.integers(0, g.nrows, k).astype(np.int32)
             src[0] = src[-1]  # a duplicate
             lv, nl = gbo.bfs_multisource(A, src)
