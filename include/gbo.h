@@ -194,7 +194,11 @@ int gbo_csr2bsr_batch(int bs, int dir, const int32_t* h_rowptr, const int32_t* h
 int gbo_csr2bsr_nnzb_dev(int bs, const int32_t* d_rowptr, const int32_t* d_colind,
                          int32_t nrows, int32_t ncols, int32_t* d_bsrRowPtr,
                          int64_t* nblocks, gbo_stream_t stream);
-/* Device CSR: fills colind/tiles given d_bsrRowPtr from gbo_csr2bsr_nnzb_dev. */
+/* Device CSR: fills colind/tiles given d_bsrRowPtr from gbo_csr2bsr_nnzb_dev.
+ * The rows are checked again (GBO_EINVAL for a column out of range or block
+ * columns that go down inside a row), and a d_bsrRowPtr whose last entry is
+ * not this CSR's tile count is refused before anything is written
+ * (synchronizes the stream). */
 int gbo_csr2bsr_fill_dev(int bs, int dir, const int32_t* d_rowptr, const int32_t* d_colind,
                          int32_t nrows, int32_t ncols, const int32_t* d_bsrRowPtr,
                          int32_t* d_bsrColInd, void* d_tiles, gbo_stream_t stream);
