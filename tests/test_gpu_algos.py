@@ -323,6 +323,31 @@ def test_bfs_multisource(gbo, po, graph_cases, bs):
             assert nl == int(lv.max()) + 1, (name, k)
 
 
+def _fan_graph(gbo, m=300):
+    """Vertex 0 joined to vertices 1..m, vertex m+1 joined to the same m:
+    pulling m+1 reads m set bits of one tile row inside one 64-tile block
+    (more than the 128 a per-row list holds in one pass), plus a tail."""
+    from graphs import sym_csr
+    a = np.arange(1, m + 1)
+    u = np.concatenate([np.zeros(m, np.int64), np.full(m, m + 1), [m + 1, m + 2]])
+    v = np.concatenate([a, a, [m + 2, m + 3]])
+    rp, ci = sym_csr(m + 4, u, v)
+    return gbo.Csr(m + 4, m + 4, rp, ci)
+
+
+def test_bfs_multisource_long_rows(gbo, po):
+    """bs 8 batched BFS with a tile row of 300 frontier bits in one block
+    (three passes over its list) and 1, 150 and 256 sources."""
+    g = _fan_graph(gbo)
+    A = gbo.BsrMatrix.from_csr(g, 8)
+    rng = np.random.default_rng(4)
+    for k in (1, 150, 256):
+        src = np.concatenate([[0], rng.integers(0, g.nrows, k - 1)]).astype(np.int32)
+        lv, _ = gbo.bfs_multisource(A, src)
+        for s in range(k):
+            assert np.array_equal(lv[s], po.bfs(g.nrows, g.rowptr, g.colind, int(src[s]))), (k, s)
+
+
 def _ppr_case(gbo, seed, n_seeds=20, K=8):
     g = gbo.gen_rmat(9, 8, seed=seed)
     rng = np.random.default_rng(seed)
@@ -341,6 +366,16 @@ def test_ppr_bs8_kernels_give_the_same_bits(gbo):
     fixed number of forward and adjoint iterations gives the same bits when
     S (forward) or W (adjoint) is moved 4 bytes off alignment."""
     g, S, W = _ppr_case(gbo, 7, n_seeds=30, K=64)
+    _ppr_kernels_agree(gbo, g, S, W)
+    # a tile row with 300 set bits in one 64-tile block (several list passes)
+    f = _fan_graph(gbo)
+    rng = np.random.default_rng(8)
+    Sf = rng.dirichlet(np.full(f.nrows, 0.5), 64).T.copy()
+    Wf = rng.standard_normal((f.nrows, 64))
+    _ppr_kernels_agree(gbo, f, Sf, Wf)
+
+
+def _ppr_kernels_agree(gbo, g, S, W):
     n, K = g.nrows, 64
     A = gbo.BsrMatrix.from_csr(g, 8)
     d_od = gbo.DeviceBuffer.from_numpy(g.outdeg().astype(np.int32))
