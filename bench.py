@@ -595,17 +595,11 @@ def main():
         # autotune: chunk size and (bs 8/16) kernel variant, best time kept
         tried = {}
         variants = [int(v) for v in args.variants.split(",")] if bs <= 16 else [0]
-        if bs == 16:  # bs 16 count under variant 3 runs variant 0
-            variants = [v for v in variants if v != 3]
         for variant in variants:
             for chunk in ((0,) if args.profile else (0, 2 * gbo_default_chunk(bs), gbo_default_chunk(bs) // 2)):
                 A.plan(chunk, variant)
                 tried[(variant, A.s.plan_chunk)] = time_count(gbo, A, dx, dy, args.steps, args.warmup, timer)
         (bv, bc), ms = min(tried.items(), key=lambda kv: kv[1])
-        if bs == 16 and bv == 0:
-            # variant 3's count at bs 16 is variant 0's kernel; keeping 3 gives the
-            # drivers below (CC, SSSP) the set-bit list min / min-plus
-            bv = 3
         A.plan(bc, bv)
         log(f"[bench] bs={bs} tuning (variant, chunk) -> ms: " + ", ".join(f"{k}: {v:.3f}" for k, v in tried.items()))
         byt = sweep_bytes(A, n)

@@ -69,12 +69,31 @@ __device__ __forceinline__ word_t<BS> ld_value_tile(const word_t<BS>* p) {
   else return *p;
 }
 
+// Read of p[i] for a wave-uniform i from memory no thread writes during the
+// kernel: through a constant-address-space pointer the load is a scalar
+// (s_load) one, off the vector memory path.
+template <typename T>
+__device__ __forceinline__ T ld_uniform(const T* p, int64_t i) {
+  return ((const __attribute__((address_space(4))) T*)p)[i];
+}
+
+// Per-tile (or per-block-column) read at a tile index that is wave-uniform
+// at bs 64 (one tile per wave instruction): a scalar load there, a vector
+// load at the smaller block sizes, where a wave spans several tiles.
+template <int BS, typename T>
+__device__ __forceinline__ T ld_tile_meta(const T* p, int64_t i) {
+  if constexpr (BS == 64) return ld_uniform(p, i);
+  else return p[i];
+}
+
 template <int BS, int OP>
 __global__ __launch_bounds__(256) void k_spmv(const SpmvArgs a) {
   using W = word_t<BS>;
   constexpr int G = kWave / BS;
   const int lane = lane_id();
-  const int g = lane / BS, r = lane % BS;
+  // at bs 64 the group index is the constant 0, so tile indices stay
+  // wave-uniform for the scalar loads below
+  const int g = BS == 64 ? 0 : lane / BS, r = lane % BS;
   const int c = wave_index();
   if (c >= a.nchunks) return;  // wave-uniform
   const int64_t cb = (int64_t)c * a.chunk;
@@ -108,7 +127,39 @@ __global__ __launch_bounds__(256) void k_spmv(const SpmvArgs a) {
       const bool full = (rs >= cb) && (re <= ce);
       const int64_t i = (int64_t)R * BS + r;
 
-      if constexpr (OP == OP_COUNT) {
+      if constexpr (OP == OP_COUNT && BS == 64) {
+        // One tile per wave instruction, so the tile index, its column index
+        // and its x word are wave-uniform: read through constant-address-
+        // space pointers they become scalar loads (one s_load_dwordx8 for
+        // eight column indices, one s_load_dwordx2 per x word) and the vector
+        // memory path carries only the tile stream.  R-MAT s22: 5.66 ->
+        // 4.19 ms (32 VGPRs; sixteen tiles in flight were no faster).
+        const W* __restrict__ xb = (const W*)a.xbits;
+        int acc = 0;
+        int64_t t = s;
+        for (; t + 7 < te; t += 8) {
+          int32_t cc[8];
+          W ww[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            cc[u] = ld_uniform(colind, t + u);
+            ww[u] = __builtin_nontemporal_load(&tiles[(t + u) * BS + r]);
+          }
+#pragma unroll
+          for (int u = 0; u < 8; ++u) acc += popcw((W)(ww[u] & ld_uniform(xb, cc[u])));
+        }
+        for (; t < te; ++t)
+          acc += popcw((W)(__builtin_nontemporal_load(&tiles[t * BS + r]) & ld_uniform(xb, ld_uniform(colind, t))));
+        if (i < a.nrows) {
+          int32_t* y = (int32_t*)a.out;
+          if (full) {
+            if (a.accumulate) y[i] += acc;
+            else y[i] = acc;
+          } else if (acc) {
+            atomicAdd(&y[i], acc);
+          }
+        }
+      } else if constexpr (OP == OP_COUNT) {
         const W* __restrict__ xb = (const W*)a.xbits;
         // Tiles are streamed once: nontemporal loads keep L2 for x.  Eight
         // tiles per lane are in flight before the dependent x gathers.
@@ -151,6 +202,10 @@ __global__ __launch_bounds__(256) void k_spmv(const SpmvArgs a) {
         // With a sparse frontier the column indices go first, then the
         // frontier words, and only the tiles whose frontier word is nonzero
         // are read (group-uniform).
+        // At bs 64 the column indices and frontier words are wave-uniform
+        // and are read with scalar loads (as in the count).
+        auto col_at = [&](int64_t k) -> int32_t { return ld_tile_meta<BS>(colind, k); };
+        auto front_at = [&](int32_t cx) -> W { return ld_tile_meta<BS>(fb, cx); };
         int hit = 0;
         int64_t t = s + g;
         if (a.sparse_frontier) {
@@ -160,10 +215,10 @@ __global__ __launch_bounds__(256) void k_spmv(const SpmvArgs a) {
             // column and frontier-word loads unconditional (slots past the
             // row read tile s's) so each stage's loads are in flight together
 #pragma unroll
-            for (int u = 0; u < 8; ++u) cc[u] = colind[t + u * G < te ? t + u * G : s];
+            for (int u = 0; u < 8; ++u) cc[u] = col_at(t + u * G < te ? t + u * G : s);
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
-              const W f = fb[cc[u]];
+              const W f = front_at(cc[u]);
               fw[u] = t + u * G < te ? f : (W)0;
             }
 #pragma unroll
@@ -176,13 +231,13 @@ __global__ __launch_bounds__(256) void k_spmv(const SpmvArgs a) {
           W ww[8];
 #pragma unroll
           for (int u = 0; u < 8; ++u) {
-            cc[u] = colind[t + u * G];
+            cc[u] = col_at(t + u * G);
             ww[u] = tiles[(t + u * G) * BS + r];
           }
 #pragma unroll
-          for (int u = 0; u < 8; ++u) hit |= (W)(ww[u] & fb[cc[u]]) != 0;
+          for (int u = 0; u < 8; ++u) hit |= (W)(ww[u] & front_at(cc[u])) != 0;
         }
-        for (; t < te; t += G) hit |= (W)(tiles[t * BS + r] & fb[colind[t]]) != 0;
+        for (; t < te; t += G) hit |= (W)(tiles[t * BS + r] & front_at(col_at(t))) != 0;
         const uint64_t bal = fold_groups<BS>(__ballot(hit));  // row r: any of its lane groups hit
         const uint64_t word = brev_low<BS>(bal) & ~(uint64_t)vword & low_mask<BS>();
         if (lane == 0) {
@@ -212,10 +267,11 @@ __global__ __launch_bounds__(256) void k_spmv(const SpmvArgs a) {
           // tile is not read (group-uniform)
           bool act[4];
 #pragma unroll
-          for (int u = 0; u < 4; ++u) cc[u] = colind[t + u * G < te ? t + u * G : s];
+          for (int u = 0; u < 4; ++u) cc[u] = ld_tile_meta<BS>(colind, t + u * G < te ? t + u * G : s);
 #pragma unroll
           for (int u = 0; u < 4; ++u) {
-            const uint64_t aw = a.active ? a.active[cc[u] >> 6] : ~0ull;  // cc[u] is a valid column
+            // cc[u] is a valid column
+            const uint64_t aw = a.active ? ld_tile_meta<BS>(a.active, cc[u] >> 6) : ~0ull;
             act[u] = t + u * G < te && ((aw >> (cc[u] & 63)) & 1ull);
           }
 #pragma unroll
@@ -224,7 +280,7 @@ __global__ __launch_bounds__(256) void k_spmv(const SpmvArgs a) {
             wo[u] = 0;
             if (act[u]) {
               wd[u] = (uint64_t)ld_value_tile<BS>(&tiles[(t + u * G) * BS + r]);
-              if (a.w) wo[u] = a.woff[t + u * G];
+              if (a.w) wo[u] = ld_tile_meta<BS>(a.woff, t + u * G);
             }
           }
 #pragma unroll
@@ -301,7 +357,7 @@ __global__ __launch_bounds__(256) void k_spmv(const SpmvArgs a) {
             // flight together
             const int64_t tu = t + u * G;
             const int64_t tc = tu < te ? tu : s;
-            cc[u] = colind[tc];
+            cc[u] = ld_tile_meta<BS>(colind, tc);
             const uint64_t wv = (uint64_t)ld_value_tile<BS>(&tiles[tc * BS + r]);
             wd[u] = tu < te ? wv : 0ull;
           }
@@ -343,7 +399,7 @@ __global__ __launch_bounds__(256) void k_spmv(const SpmvArgs a) {
             // flight together
             const int64_t tu = t + u * G;
             const int64_t tc = tu < te ? tu : s;
-            cc[u] = colind[tc];
+            cc[u] = ld_tile_meta<BS>(colind, tc);
             const uint64_t wv = (uint64_t)ld_value_tile<BS>(&tiles[tc * BS + r]);
             wd[u] = tu < te ? wv : 0ull;
           }
@@ -1063,6 +1119,107 @@ __global__ __launch_bounds__(256) void k_count8(const SpmvArgs a) {
   if (open) finish_row();  // a row that continues past the chunk: atomic partial
 }
 
+// bs 16 count (plan variant 3) with four lanes per tile: lane l takes tile
+// slot l / 4 of a step (16 tiles, one 512-byte contiguous read per wave
+// instruction) and rows 4q..4q+3 (q = l % 4) of it as one 8-byte word; x is
+// replicated into the four 16-bit fields and each field's popcount added to
+// the lane's row counter, which are summed over the 16 slots at the end of a
+// block row.  Against the row-group kernel (16 lanes of 2 bytes per tile):
+// R-MAT s22 0.884 ms against 0.954, same counts; two lanes per tile (8
+// counters per lane) took 0.996.  Rows inside the chunk are written as one
+// 16-byte store per lane (scattered 4-byte stores tripled the written bytes).
+constexpr int kCount16Step = 4;  // tiles per lane in flight
+__global__ __launch_bounds__(256) void k_count16q(const SpmvArgs a) {
+  constexpr int BS = 16, NU = kCount16Step, SL = kWave / 4;  // tile slots per step
+  const int lane = lane_id();
+  const int slot = lane >> 2, q = lane & 3;
+  const int c = wave_index();
+  if (c >= a.nchunks) return;  // wave-uniform
+  const int64_t cb = (int64_t)c * a.chunk;
+  const int64_t ce = min(cb + (int64_t)a.chunk, a.T);
+  const bool last = (c == a.nchunks - 1);
+  const uint64_t* __restrict__ tw = (const uint64_t*)a.tiles;  // 4 words per tile
+  const uint16_t* __restrict__ xb = (const uint16_t*)a.xbits;
+  const int32_t* __restrict__ colind = a.colind;
+  int32_t* __restrict__ y = (int32_t*)a.out;
+  const bool vec_y = ((uintptr_t)y & 15) == 0;
+  for (int32_t R0 = a.plan[c]; R0 < a.nbr;) {
+    const int32_t Rl = R0 + lane;
+    int32_t rsl = 0, rel = 0;
+    if (Rl < a.nbr) {
+      rsl = a.rowptr[Rl];
+      rel = a.rowptr[Rl + 1];
+    }
+    const bool belong = (Rl < a.nbr) && (last || (int64_t)rsl < ce);
+    const uint64_t bmask = __ballot(belong);
+    if (belong && rel == rsl) write_empty_row<BS, OP_COUNT>(a, Rl);
+    uint64_t work = __ballot(belong && rel > rsl);
+    while (work) {
+      const int j = __builtin_ctzll(work);
+      work &= work - 1;
+      const int32_t R = R0 + j;
+      const int64_t rs = __builtin_amdgcn_readlane(rsl, j), re = __builtin_amdgcn_readlane(rel, j);
+      const int64_t s = rs > cb ? rs : cb;
+      const int64_t te = re < ce ? re : ce;
+      const bool full = (rs >= cb) && (re <= ce);
+      uint32_t cnt[4] = {0u, 0u, 0u, 0u};
+      for (int64_t t0 = s; t0 < te; t0 += (int64_t)SL * NU) {
+        uint64_t w[NU];
+        int32_t cc[NU];
+#pragma unroll
+        for (int u = 0; u < NU; ++u) {
+          const int64_t t = t0 + (int64_t)u * SL + slot;
+          const bool ok = t < te;
+          const int64_t tc = ok ? t : s;  // in range: the row's first tile
+          cc[u] = colind[tc];
+          const uint64_t wv = __builtin_nontemporal_load(&tw[tc * 4 + q]);
+          w[u] = ok ? wv : 0ull;
+        }
+#pragma unroll
+        for (int u = 0; u < NU; ++u) {
+          const uint64_t xx = (uint64_t)xb[(uint32_t)cc[u]] * 0x0001000100010001ull;
+          const uint64_t m = w[u] & xx;
+          cnt[0] += (uint32_t)__popc((uint32_t)m & 0xFFFFu);
+          cnt[1] += (uint32_t)__popc((uint32_t)m >> 16);
+          cnt[2] += (uint32_t)__popc((uint32_t)(m >> 32) & 0xFFFFu);
+          cnt[3] += (uint32_t)__popc((uint32_t)(m >> 48));
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        for (int off = 4; off < kWave; off <<= 1) cnt[k] += (uint32_t)__shfl_xor((int)cnt[k], off);
+      if (slot == 0) {
+        const int64_t i0 = (int64_t)R * BS + 4 * q;
+        if (full && vec_y && i0 + 4 <= a.nrows) {
+          // one 16-byte store per lane: the 4 lanes write the row's 64 bytes
+          int4* p = (int4*)(y + i0);
+          int4 v = make_int4((int)cnt[0], (int)cnt[1], (int)cnt[2], (int)cnt[3]);
+          if (a.accumulate) {
+            const int4 o = *p;
+            v.x += o.x; v.y += o.y; v.z += o.z; v.w += o.w;
+          }
+          *p = v;
+        } else {
+#pragma unroll
+          for (int k = 0; k < 4; ++k) {
+            const int64_t i = i0 + k;
+            if (i < a.nrows) {
+              if (full) {
+                if (a.accumulate) y[i] += (int32_t)cnt[k];
+                else y[i] = (int32_t)cnt[k];
+              } else if (cnt[k]) {
+                atomicAdd(&y[i], (int32_t)cnt[k]);
+              }
+            }
+          }
+        }
+      }
+    }
+    if (__popcll(bmask) < kWave) break;
+    R0 += kWave;
+  }
+}
+
 // bs 8/16 min / min-plus, lane per tile (plan variant 3).  A wave takes its
 // chunk 64 block rows (a window) at a time and the window's tiles 64 per
 // round (one per lane: more tiles per lane made the kernel register-bound
@@ -1331,23 +1488,29 @@ __global__ __launch_bounds__(256) void k_sum_fixup(const SpmvArgs a) {
   if (i < a.nrows) ((float*)a.out)[i] = (float)((double)a.scale * s + (double)a.bias);
 }
 
-// Dense bs 8 BFS step, second half: next[R] = rows of block row R whose
+// Dense bs 8/16 BFS step, second half: next[R] = rows of block row R whose
 // frontier-neighbour count is nonzero, minus the visited ones; new bits are
 // counted into the slots.
-__global__ __launch_bounds__(256) void k_bfs_pack8(const int32_t* __restrict__ cnt, const uint8_t* __restrict__ vis,
-                                                   uint8_t* __restrict__ next, int32_t nbr, int32_t nrows,
-                                                   unsigned long long* __restrict__ slots) {
+template <int BS>
+__global__ __launch_bounds__(256) void k_bfs_pack(const int32_t* __restrict__ cnt, const word_t<BS>* __restrict__ vis,
+                                                  word_t<BS>* __restrict__ next, int32_t nbr, int32_t nrows,
+                                                  unsigned long long* __restrict__ slots) {
   const int64_t R = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   uint32_t word = 0;
   if (R < nbr) {
-    const int4* p = (const int4*)(cnt + R * 8);
-    const int4 lo = p[0], hi = p[1];
-    const int32_t c8[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+    const int4* p = (const int4*)(cnt + R * BS);
 #pragma unroll
-    for (int r = 0; r < 8; ++r)
-      if (R * 8 + r < nrows && c8[r] > 0) word |= 1u << (7 - r);
+    for (int h = 0; h < BS / 4; ++h) {
+      const int4 v = p[h];
+      const int32_t c4[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int r = 4 * h + k;
+        if (R * BS + r < nrows && c4[k] > 0) word |= 1u << (BS - 1 - r);
+      }
+    }
     word &= ~(uint32_t)vis[R];
-    next[R] = (uint8_t)word;
+    next[R] = (word_t<BS>)word;
   }
   uint64_t added = (uint64_t)__popc(word);
   for (int off = 1; off < kWave; off <<= 1) added += __shfl_xor(added, off);
@@ -1360,26 +1523,26 @@ static int spmv_launch_t(const SpmvArgs& a, hipStream_t st) {
   // column indices: they need 16-byte-aligned arrays, otherwise variant 0
   // runs.
   const bool aligned = ((((uintptr_t)a.tiles) | ((uintptr_t)a.colind)) & 15) == 0;
-  // variant 3: bs 8 count is k_count8; bs 16 count runs variant 0 (its
-  // fastest count kernel)
+  // variant 3: bs 8 count is k_count8, bs 16 count k_count16q
   const bool lean8 = (BS == 8 && OP == OP_COUNT && a.variant == 3 && aligned);
+  const bool quad16 = (BS == 16 && OP == OP_COUNT && a.variant == 3 && aligned);
   const bool stream = (BS <= 16 && OP == OP_COUNT && a.variant == 2 && aligned);
   // variant 3 at bs 8 is the lean count kernel; its BFS step is the
   // lane-per-tile kernel (faster than the row-group one at bs 8)
   const bool lpt = ((BS <= 16 && (OP == OP_COUNT || OP == OP_BFS) && a.variant == 1) ||
                     (BS == 8 && OP == OP_BFS && a.variant == 3)) &&
                    !(OP == OP_BFS && a.sparse_frontier);
-  if constexpr (BS == 8 && OP == OP_BFS) {
+  if constexpr ((BS == 8 || BS == 16) && OP == OP_BFS) {
     if (a.variant == 3 && a.bfs_counts && aligned && !a.sparse_frontier) {
-      // dense step: the lean count kernel gives each row's number of
+      // dense step: the variant 3 count kernel gives each row's number of
       // frontier neighbours (every tile is read; no visited-row skip)
       SpmvArgs c = a;
       c.out = a.bfs_counts;
       c.accumulate = 0;
-      int rc = spmv_launch_t<8, OP_COUNT>(c, st);
+      int rc = spmv_launch_t<BS, OP_COUNT>(c, st);
       if (rc) return rc;
-      k_bfs_pack8<<<(unsigned)(((int64_t)a.nbr + 255) / 256), 256, 0, st>>>(
-          a.bfs_counts, (const uint8_t*)a.visited, (uint8_t*)a.out, a.nbr, a.nrows, a.count_slots);
+      k_bfs_pack<BS><<<(unsigned)(((int64_t)a.nbr + 255) / 256), 256, 0, st>>>(
+          a.bfs_counts, (const word_t<BS>*)a.visited, (word_t<BS>*)a.out, a.nbr, a.nrows, a.count_slots);
       GBO_LAUNCH_CHECK();
       return GBO_OK;
     }
@@ -1392,7 +1555,9 @@ static int spmv_launch_t(const SpmvArgs& a, hipStream_t st) {
   }
   const int64_t blocks = ((int64_t)a.nchunks + 3) / 4;
   if constexpr (BS <= 16 && OP == OP_COUNT) {
-    if (lean8) {
+    if (quad16) {
+      k_count16q<<<(unsigned)blocks, 256, 0, st>>>(a);
+    } else if (lean8) {
       // steps of 512 tiles for chunks of 512 or more, else 256
       if constexpr (BS == 8) {
         if (a.chunk >= 512) k_count8<4><<<(unsigned)blocks, 256, 0, st>>>(a);
@@ -1534,7 +1699,7 @@ int gbo_bsrbmv_bfs_step(const gbo_bsr* A, const void* d_frontier, const void* d_
   a.visited = d_visited;
   a.out = d_next;
   a.count_slots = g.pv.slots;
-  if (A->bs == 8 && a.variant == 3) a.bfs_counts = g.pv.bfs_counts;
+  if (A->bs <= 16 && a.variant == 3) a.bfs_counts = g.pv.bfs_counts;
   rc = spmv_launch(A->bs, OP_BFS, a, stream);
   if (rc == GBO_OK) rc = count_fold(g.pv.slots, d_count, stream);
   return rc;

@@ -19,8 +19,8 @@ namespace gbo {
 //   slots       kCountSlots x u64   BFS frontier-size slots (zero between calls)
 //   count       1 x int64           the folded frontier size
 //   partial     nchunks*2*bs x f64  split-row partials of the FP32 sum
-//   bfs_counts  8*(nbr+1) x int32   per-row counts of the bs 8 dense BFS step
-//                                   (bs 8 only)
+//   bfs_counts  bs*(nbr+1) x int32  per-row counts of the dense BFS step
+//                                   (bs 8 and 16 only)
 // Calls on one matrix therefore share that scratch: they must be ordered
 // (one stream, or synchronised); separately created matrices do not share.
 struct PlanLayout {
@@ -37,7 +37,7 @@ struct PlanView {
   unsigned long long* slots = nullptr;
   int64_t* count = nullptr;
   double* partial = nullptr;
-  int32_t* bfs_counts = nullptr;  // null unless bs 8
+  int32_t* bfs_counts = nullptr;  // null unless bs 8 or 16
 };
 
 int default_chunk(int bs);
@@ -113,7 +113,7 @@ struct SpmvArgs {
   const int64_t* woff;  // MINPLUS per-tile weight offsets
   const void* xval;     // MINPLUS dist / SUM x / MIN x
   unsigned long long* count_slots;  // BFS: newly set bits, summed per slot (kCountSlots, zero on entry)
-  int32_t* bfs_counts;  // BFS, bs 8 variant 3 (nullable): int32 scratch of nbr*8; when set the step runs
+  int32_t* bfs_counts;  // BFS, bs 8/16 variant 3 (nullable): int32 scratch of nbr*bs; when set the step runs
                         // the count kernel into it and packs the rows with a nonzero count (dense step)
   double* partial;      // SUM: split-row partials [nchunks][2][bs]
   const uint64_t* active;  // MINPLUS: optional per-block-column activity bits (nullable)

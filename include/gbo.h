@@ -347,11 +347,13 @@ int gbo_bfs(const gbo_bsr* AT, int32_t source, int32_t* d_level,
  * cannot allocate device memory during the call (serving loops, harnesses
  * that forbid hipMalloc).  Same arguments and results as gbo_bfs (pull BFS
  * over AT); AT must already have a plan (gbo_bsr_plan), else GBO_EINVAL.
- * The frontier / next / visited words, the frontier-size counters and (bs 8,
- * variant 3) the dense step's row counts live in one static device array of
- * 32 MiB held by each GPU the library runs on until the process ends; a
- * graph whose scratch exceeds it is refused with GBO_ERANGE before any
- * launch (bs 8: up to about 7.6M vertices; bs 16-64: up to about 89M).
+ * The frontier / next / visited words and the frontier-size counters live
+ * in one static device array of 32 MiB held by each GPU the library runs on
+ * until the process ends; a graph whose scratch exceeds it (about 89M
+ * vertices at any block size) is refused with GBO_ERANGE before any launch.
+ * At bs 8/16 (variant 3) the dense step's row counts (4 bytes a vertex) go
+ * there too for graphs up to about 7.6M vertices; larger ones take the
+ * visited-row-skipping step at every level (same levels).
  * Calls that overlap on one GPU share that array and race: order them (one
  * stream, or synchronise between calls). */
 int gbo_bfs_noalloc(const gbo_bsr* AT, int32_t source, int32_t* d_level,

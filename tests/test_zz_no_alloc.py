@@ -48,11 +48,30 @@ def test_noalloc_matches_gbo_bfs(gbo, bs):
         A.free()
 
 
-def test_noalloc_refuses(gbo):
-    # 8M vertices, no edges: bs 8's words and dense-step counts exceed the array
+@pytest.mark.parametrize("bs", (8, 16))
+def test_noalloc_without_dense_counts(gbo, bs):
+    # 8M vertices, one edge into vertex 0 from 8M-1 (symmetric): the frontier
+    # words fit the array, the dense step's row counts (32 MiB) do not, so
+    # every level takes the visited-row-skipping step; levels as gbo_bfs
     n = 1 << 23
+    rowptr = np.zeros(n + 1, np.int32)
+    rowptr[1:] = 1
+    rowptr[n] = 2
+    g = gbo.Csr(n, n, rowptr, np.array([n - 1, 0], np.int32))
+    A = gbo.BsrMatrix.from_csr(g, bs)
+    want, want_sizes = gbo.bfs(A, 0)
+    lv, sizes = _run_noalloc(gbo, A, 0)
+    assert np.array_equal(lv, want) and np.array_equal(sizes, want_sizes)
+    assert lv[0] == 0 and lv[n - 1] == 1 and int((lv >= 0).sum()) == 2
+    A.free()
+
+
+def test_noalloc_refuses(gbo):
+    # 100M vertices, no edges: the frontier / next / visited words (37.5 MB
+    # at bs 64) exceed the array
+    n = 100_000_000
     g = gbo.Csr(n, n, np.zeros(n + 1, np.int32), np.zeros(0, np.int32))
-    A = gbo.BsrMatrix.from_csr(g, 8)
+    A = gbo.BsrMatrix.from_csr(g, 64)
     with pytest.raises(gbo.GboError) as e:
         _run_noalloc(gbo, A, 0)
     assert e.value.code == -6 and "static array" in str(e.value)
