@@ -123,9 +123,9 @@ __global__ __launch_bounds__(256) void k_popc_words(const word_t<BS>* __restrict
 
 // Scratch of a BFS call supplied by the caller of bfs_t instead of being
 // allocated: frontier / next / visited words (3 x nbr words), the frontier-
-// size slots (kCountSlots, cleared on entry) and count, and (bs 8/16,
-// variant 3, when they fit) the dense step's row counts (bs x (nbr + 1)
-// int32, else null).
+// size slots (kCountSlots, cleared on entry) and count, and (where
+// dense_bfs_step holds and they fit) the dense step's row counts (bs x
+// (nbr + 1) int32, else null).
 struct BfsScratch {
   void* words;
   unsigned long long* slots;
@@ -181,9 +181,10 @@ static int bfs_t(const gbo_bsr* A, const gbo_bsr* Aout, int32_t source, int32_t*
   a.nchunks = pv.nchunks;
   a.variant = pv.variant;
   a.count_slots = pv.slots;
-  // bs 8/16 variant 3: levels while fewer than half the vertices are visited
-  // take the dense step (count kernel + pack); later ones skip visited rows
-  int32_t* dense_counts = (BS <= 16 && pv.variant == 3) ? pv.bfs_counts : nullptr;
+  // bs 8/16 variant 3 and bs 32: levels while fewer than half the vertices
+  // are visited take the dense step (count kernel + pack); later ones skip
+  // visited rows
+  int32_t* dense_counts = dense_bfs_step(BS, pv.variant) ? pv.bfs_counts : nullptr;
   int64_t visited = 1;
   int32_t lvl = 0;
   int64_t fsize = 1;  // vertices in the current frontier
@@ -429,11 +430,12 @@ static int noalloc_layout(const gbo_bsr* AT, unsigned char* base, BfsScratch* s)
   const size_t o_count = align256(sizeof(unsigned long long) * kCountSlots);
   const size_t o_words = o_count + 256;
   const size_t o_dense = o_words + align256(wbytes);
-  // bs 8/16, variant 3: the dense step's row counts (4 bytes a vertex) when
-  // they fit as well; without them every level takes the step that skips
-  // visited rows (same levels)
+  // the dense step's row counts (4 bytes a vertex) when they fit as well;
+  // without them every level takes the step that skips visited rows (same
+  // levels)
   const size_t dbytes = sizeof(int32_t) * (size_t)AT->bs * ((size_t)AT->nblockrows + 1);
-  const bool dense = AT->bs <= 16 && AT->plan_variant == 3 && o_dense + align256(dbytes) <= kNoAllocScratchBytes;
+  const bool dense =
+      dense_bfs_step(AT->bs, AT->plan_variant) && o_dense + align256(dbytes) <= kNoAllocScratchBytes;
   const size_t need = o_dense + (dense ? align256(dbytes) : 0);
   if (need > kNoAllocScratchBytes)
     return set_error(GBO_ERANGE,
