@@ -1346,10 +1346,16 @@ __global__ __launch_bounds__(256) void k_value_lpt(const SpmvArgs a) {
   using Acc = typename std::conditional<SUM, double, uint32_t>::type;
   constexpr int NWD = BS == 8 ? 1 : 4;  // 64-bit words per tile
   constexpr uint32_t kFlip = 0x80000000u;
+  // block rows per window: the window's accumulators are WIN * BS values in
+  // LDS per wave.  At bs 16 a 64-row window made the sum's workgroup hold
+  // 38 KB of LDS (4 waves per SIMD); 32 rows take it to 22.5 KB (7 waves):
+  // R-MAT s22 sum 1.38 -> 1.09 ms, weighted min-plus 1.47 -> 1.40 ms.  At
+  // bs 8 smaller windows were no faster.
+  constexpr int WIN = BS == 16 ? 32 : kWave;
   __shared__ uint32_t s_idx[4][kValCap];
   __shared__ uint32_t s_widx[4][PLUS ? kValCap : 1];
   __shared__ uint16_t s_dst[4][kValCap];
-  __shared__ Acc s_acc[4][kWave * BS];
+  __shared__ Acc s_acc[4][WIN * BS];
   const int lane = lane_id();
   const int wv = threadIdx.x / kWave;
   const int c = wave_index();
@@ -1372,14 +1378,14 @@ __global__ __launch_bounds__(256) void k_value_lpt(const SpmvArgs a) {
       rsl = a.rowptr[Rl];
       rel = a.rowptr[Rl + 1];
     }
-    const bool belong = (Rl < a.nbr) && (last || (int64_t)rsl < ce);
+    const bool belong = lane < WIN && (Rl < a.nbr) && (last || (int64_t)rsl < ce);
     const int nb = __popcll(__ballot(belong));  // rows R0 .. R0+nb-1 (a prefix)
     if (nb == 0) break;
     if (belong && rel == rsl) write_empty_row<BS, OP>(a, Rl);
     const int32_t relx = belong ? rel : INT32_MAX;
     const int64_t w0 = max((int64_t)__builtin_amdgcn_readlane(rsl, 0), cb);
     const int64_t w1 = min((int64_t)__builtin_amdgcn_readlane(rel, nb - 1), ce);
-    for (int k = lane; k < kWave * BS; k += kWave) acc[k] = SUM ? (Acc)0 : (Acc)0xFFFFFFFFu;
+    for (int k = lane; k < WIN * BS; k += kWave) acc[k] = SUM ? (Acc)0 : (Acc)0xFFFFFFFFu;
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     for (int64_t t0 = w0; t0 < w1; t0 += kWave) {
       const int64_t t = t0 + lane;
@@ -1528,8 +1534,8 @@ __global__ __launch_bounds__(256) void k_value_lpt(const SpmvArgs a) {
       }
     }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    if (nb < kWave) break;
-    R0 += kWave;
+    if (nb < WIN) break;
+    R0 += WIN;
   }
 }
 
