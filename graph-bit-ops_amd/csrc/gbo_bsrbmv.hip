@@ -1119,6 +1119,17 @@ __global__ __launch_bounds__(256) void k_count8(const SpmvArgs a) {
   if (open) finish_row();  // a row that continues past the chunk: atomic partial
 }
 
+// Row l / 4 * 4 + l % 4 of a block row whose lane j (j < 16) holds the totals
+// of rows 4j..4j+3 in c[0..3]: lane l reads c[l % 4] of lane l / 4.  The
+// reads are unconditional (every lane takes part in the shuffles).
+__device__ __forceinline__ uint32_t row_of_quad(const uint32_t* c, int lane) {
+  const int src = (lane >> 2) & 15;
+  const uint32_t v0 = (uint32_t)__shfl((int)c[0], src), v1 = (uint32_t)__shfl((int)c[1], src);
+  const uint32_t v2 = (uint32_t)__shfl((int)c[2], src), v3 = (uint32_t)__shfl((int)c[3], src);
+  const int k = lane & 3;
+  return k == 0 ? v0 : k == 1 ? v1 : k == 2 ? v2 : v3;
+}
+
 // bs 16 count (plan variant 3) with four lanes per tile: lane l takes tile
 // slot l / 4 of a step (16 tiles, one 512-byte contiguous read per wave
 // instruction) and rows 4q..4q+3 (q = l % 4) of it as one 8-byte word; x is
@@ -1126,8 +1137,10 @@ __global__ __launch_bounds__(256) void k_count8(const SpmvArgs a) {
 // the lane's row counter, which are summed over the 16 slots at the end of a
 // block row.  Against the row-group kernel (16 lanes of 2 bytes per tile):
 // R-MAT s22 0.884 ms against 0.954, same counts; two lanes per tile (8
-// counters per lane) took 0.996.  Rows inside the chunk are written as one
-// 16-byte store per lane (scattered 4-byte stores tripled the written bytes).
+// counters per lane) took 0.996.  The row totals are moved to lanes 0..15
+// before they are written: four strided store or atomic instructions per
+// block row had tripled the written bytes (split rows' atomics are not
+// coalesced).
 constexpr int kCount16Step = 4;  // tiles per lane in flight
 __global__ __launch_bounds__(256) void k_count16q(const SpmvArgs a) {
   constexpr int BS = 16, NU = kCount16Step, SL = kWave / 4;  // tile slots per step
@@ -1142,7 +1155,6 @@ __global__ __launch_bounds__(256) void k_count16q(const SpmvArgs a) {
   const uint16_t* __restrict__ xb = (const uint16_t*)a.xbits;
   const int32_t* __restrict__ colind = a.colind;
   int32_t* __restrict__ y = (int32_t*)a.out;
-  const bool vec_y = ((uintptr_t)y & 15) == 0;
   for (int32_t R0 = a.plan[c]; R0 < a.nbr;) {
     const int32_t Rl = R0 + lane;
     int32_t rsl = 0, rel = 0;
@@ -1188,30 +1200,17 @@ __global__ __launch_bounds__(256) void k_count16q(const SpmvArgs a) {
 #pragma unroll
       for (int k = 0; k < 4; ++k)
         for (int off = 4; off < kWave; off <<= 1) cnt[k] += (uint32_t)__shfl_xor((int)cnt[k], off);
-      if (slot == 0) {
-        const int64_t i0 = (int64_t)R * BS + 4 * q;
-        if (full && vec_y && i0 + 4 <= a.nrows) {
-          // one 16-byte store per lane: the 4 lanes write the row's 64 bytes
-          int4* p = (int4*)(y + i0);
-          int4 v = make_int4((int)cnt[0], (int)cnt[1], (int)cnt[2], (int)cnt[3]);
-          if (a.accumulate) {
-            const int4 o = *p;
-            v.x += o.x; v.y += o.y; v.z += o.z; v.w += o.w;
-          }
-          *p = v;
-        } else {
-#pragma unroll
-          for (int k = 0; k < 4; ++k) {
-            const int64_t i = i0 + k;
-            if (i < a.nrows) {
-              if (full) {
-                if (a.accumulate) y[i] += (int32_t)cnt[k];
-                else y[i] = (int32_t)cnt[k];
-              } else if (cnt[k]) {
-                atomicAdd(&y[i], (int32_t)cnt[k]);
-              }
-            }
-          }
+      // every lane with q = l % 4 now holds rows 4q..4q+3; lane l < 16 takes
+      // row l from lane l / 4, so one store (or atomic) instruction covers
+      // the block row's 64 contiguous bytes
+      const int32_t v = (int32_t)row_of_quad(cnt, lane);
+      const int64_t i = (int64_t)R * BS + lane;
+      if (lane < BS && i < a.nrows) {
+        if (full) {
+          if (a.accumulate) y[i] += v;
+          else y[i] = v;
+        } else if (v) {
+          atomicAdd(&y[i], v);
         }
       }
     }
@@ -1241,7 +1240,6 @@ __global__ __launch_bounds__(256) void k_count32q(const SpmvArgs a) {
   const uint32_t* __restrict__ xb = (const uint32_t*)a.xbits;
   const int32_t* __restrict__ colind = a.colind;
   int32_t* __restrict__ y = (int32_t*)a.out;
-  const bool vec_y = ((uintptr_t)y & 15) == 0;
   for (int32_t R0 = a.plan[c]; R0 < a.nbr;) {
     const int32_t Rl = R0 + lane;
     int32_t rsl = 0, rel = 0;
@@ -1286,29 +1284,16 @@ __global__ __launch_bounds__(256) void k_count32q(const SpmvArgs a) {
 #pragma unroll
       for (int k = 0; k < 4; ++k)
         for (int off = 8; off < kWave; off <<= 1) cnt[k] += (uint32_t)__shfl_xor((int)cnt[k], off);
-      if (slot == 0) {
-        const int64_t i0 = (int64_t)R * BS + 4 * p;
-        if (full && vec_y && i0 + 4 <= a.nrows) {
-          int4* q = (int4*)(y + i0);
-          int4 v = make_int4((int)cnt[0], (int)cnt[1], (int)cnt[2], (int)cnt[3]);
-          if (a.accumulate) {
-            const int4 o = *q;
-            v.x += o.x; v.y += o.y; v.z += o.z; v.w += o.w;
-          }
-          *q = v;
-        } else {
-#pragma unroll
-          for (int k = 0; k < 4; ++k) {
-            const int64_t i = i0 + k;
-            if (i < a.nrows) {
-              if (full) {
-                if (a.accumulate) y[i] += (int32_t)cnt[k];
-                else y[i] = (int32_t)cnt[k];
-              } else if (cnt[k]) {
-                atomicAdd(&y[i], (int32_t)cnt[k]);
-              }
-            }
-          }
+      // every lane with p = l % 8 now holds rows 4p..4p+3; lane l < 32 takes
+      // row l from lane l / 4 (one contiguous store or atomic instruction)
+      const int32_t v = (int32_t)row_of_quad(cnt, lane);
+      const int64_t i = (int64_t)R * BS + lane;
+      if (lane < BS && i < a.nrows) {
+        if (full) {
+          if (a.accumulate) y[i] += v;
+          else y[i] = v;
+        } else if (v) {
+          atomicAdd(&y[i], v);
         }
       }
     }
