@@ -1302,6 +1302,208 @@ __global__ __launch_bounds__(256) void k_count32q(const SpmvArgs a) {
   }
 }
 
+// Value products (min / min-plus / FP32 sum) with four rows per lane: lane l
+// holds rows 4p..4p+3 (p = l % (bs/4)) of tile slot l / (bs/4), so a wave
+// instruction loads 64/(bs/4) whole tiles (bs 32: 8, bs 16: 16) and a gather
+// pass issues one gather instruction per four tiles per lane instead of one
+// per tile slot of the row-group kernel.  Per pass each lane takes one set
+// bit of each of its NU tiles (rows from the last to the first, each row's
+// bits from the last column back, so a weighted tile's weight ranks count
+// down from the lane's last bit); the four row accumulators are summed over
+// the tile slots at the end of a block row (FP64 butterfly for the sum: a
+// fixed order) and moved to one lane per row for the write.
+// Used at bs 32 for min and min-plus (R-MAT s22: weighted min-plus 4.51 ->
+// 3.85 ms, unit 3.88 -> 3.00 ms, min 2.36 -> 2.30 ms against the row-group
+// kernel).  The FP32 sum stays on the row-group kernel (2.77 against 2.70
+// ms), and at bs 16 the set-bit list kernel is faster for every op (min 1.05
+// against 1.49 ms).
+template <int BS, int OP>
+__global__ __launch_bounds__(256) void k_value_q(const SpmvArgs a) {
+  static_assert(BS == 16 || BS == 32, "four rows per lane at bs 16 or 32");
+  static_assert(OP == OP_MIN || OP == OP_MINPLUS || OP == OP_SUM, "min / min-plus / sum only");
+  constexpr bool PLUS = OP == OP_MINPLUS;
+  constexpr bool SUM = OP == OP_SUM;
+  using Acc = typename std::conditional<SUM, double, uint32_t>::type;
+  constexpr int LPT = BS / 4, SL = kWave / LPT, NU = 4;
+  constexpr uint32_t kFlip = 0x80000000u;
+  const Acc ident = SUM ? (Acc)0 : (Acc)0xFFFFFFFFu;
+  const int lane = lane_id();
+  const int slot = lane / LPT, p = lane % LPT;
+  const int c = wave_index();
+  if (c >= a.nchunks) return;  // wave-uniform
+  const int64_t cb = (int64_t)c * a.chunk;
+  const int64_t ce = min(cb + (int64_t)a.chunk, a.T);
+  const bool last = (c == a.nchunks - 1);
+  const int32_t* __restrict__ colind = a.colind;
+  const uint32_t* __restrict__ xv = (const uint32_t*)a.xval;
+  const bool weighted = PLUS && a.w != nullptr;
+  for (int32_t R0 = a.plan[c]; R0 < a.nbr;) {
+    const int32_t Rl = R0 + lane;
+    int32_t rsl = 0, rel = 0;
+    if (Rl < a.nbr) {
+      rsl = a.rowptr[Rl];
+      rel = a.rowptr[Rl + 1];
+    }
+    const bool belong = (Rl < a.nbr) && (last || (int64_t)rsl < ce);
+    const uint64_t bmask = __ballot(belong);
+    if (belong && rel == rsl) write_empty_row<BS, OP>(a, Rl);
+    uint64_t work = __ballot(belong && rel > rsl);
+    while (work) {
+      const int j = __builtin_ctzll(work);
+      work &= work - 1;
+      const int32_t R = R0 + j;
+      const int64_t rs = __builtin_amdgcn_readlane(rsl, j), re = __builtin_amdgcn_readlane(rel, j);
+      const int64_t s = rs > cb ? rs : cb;
+      const int64_t te = re < ce ? re : ce;
+      const bool full = (rs >= cb) && (re <= ce);
+      Acc acc[4] = {ident, ident, ident, ident};
+      for (int64_t t0 = s; t0 < te; t0 += (int64_t)SL * NU) {
+        int32_t cc[NU];
+        bool act[NU];
+        uint32_t rw[NU][4];
+        int rank[NU];
+        int64_t wo[NU];
+        // three stages over the NU tiles so each stage's loads are in flight
+        // together: column indices (slots past the row read its first tile),
+        // activity bits (min-plus), then the words and weight offsets of the
+        // active tiles
+#pragma unroll
+        for (int u = 0; u < NU; ++u) {
+          const int64_t t = t0 + (int64_t)u * SL + slot;
+          cc[u] = colind[t < te ? t : s];
+        }
+#pragma unroll
+        for (int u = 0; u < NU; ++u) {
+          const int64_t t = t0 + (int64_t)u * SL + slot;
+          bool on = t < te;
+          if constexpr (PLUS) {
+            if (a.active) on = on && ((a.active[cc[u] >> 6] >> (cc[u] & 63)) & 1ull);
+          }
+          act[u] = on;
+        }
+#pragma unroll
+        for (int u = 0; u < NU; ++u) {
+          const int64_t t = t0 + (int64_t)u * SL + slot;
+          const int64_t tc = t < te ? t : s;
+          uint32_t r4[4] = {0u, 0u, 0u, 0u};
+          wo[u] = 0;
+          if (!PLUS || act[u]) {
+            if constexpr (BS == 32) {
+              typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+              const u32x4 v = ((const u32x4*)a.tiles)[tc * 8 + p];
+              r4[0] = v.x; r4[1] = v.y; r4[2] = v.z; r4[3] = v.w;
+            } else {
+              const uint64_t v = ((const uint64_t*)a.tiles)[tc * 4 + p];
+              r4[0] = (uint32_t)v & 0xFFFFu;
+              r4[1] = (uint32_t)v >> 16;
+              r4[2] = (uint32_t)(v >> 32) & 0xFFFFu;
+              r4[3] = (uint32_t)(v >> 48);
+            }
+            if (weighted) wo[u] = a.woff[tc];
+          }
+#pragma unroll
+          for (int k = 0; k < 4; ++k) rw[u][k] = act[u] ? r4[k] : 0u;
+        }
+        if (weighted) {
+          // weight rank of the lane's last set bit: set bits of the tile's
+          // earlier rows (lanes p' < p of the slot) plus the lane's own
+#pragma unroll
+          for (int u = 0; u < NU; ++u) {
+            const int pc = __popc(rw[u][0]) + __popc(rw[u][1]) + __popc(rw[u][2]) + __popc(rw[u][3]);
+            int incl = pc;
+#pragma unroll
+            for (int off = 1; off < LPT; off <<= 1) {
+              const int o = __shfl_up(incl, off, LPT);
+              if (p >= off) incl += o;
+            }
+            rank[u] = incl - 1;
+          }
+        }
+        while (true) {
+          bool any = false;
+#pragma unroll
+          for (int u = 0; u < NU; ++u) any |= (rw[u][0] | rw[u][1] | rw[u][2] | rw[u][3]) != 0u;
+          if (!__ballot(any)) break;  // wave-uniform
+          // one set bit of each tile; all gathers of the pass are issued
+          // before any is used
+          uint32_t gv[NU], wt[NU];
+          int kk[NU];
+          bool has[NU];
+#pragma unroll
+          for (int u = 0; u < NU; ++u) {
+            const uint32_t w0 = rw[u][0], w1 = rw[u][1], w2 = rw[u][2], w3 = rw[u][3];
+            const int k = w3 ? 3 : w2 ? 2 : w1 ? 1 : 0;
+            const uint32_t wk = k == 3 ? w3 : k == 2 ? w2 : k == 1 ? w1 : w0;
+            has[u] = wk != 0u;
+            kk[u] = k;
+            gv[u] = 0u;
+            wt[u] = 1u;
+            if (has[u]) {
+              const int b = __builtin_ctz(wk);
+              gv[u] = xv[(int64_t)cc[u] * BS + (BS - 1 - b)];
+              if (weighted) wt[u] = a.w[wo[u] + rank[u]];
+              const uint32_t nw = wk & (wk - 1u);
+#pragma unroll
+              for (int q = 0; q < 4; ++q) rw[u][q] = q == k ? nw : rw[u][q];
+              --rank[u];
+            }
+          }
+#pragma unroll
+          for (int u = 0; u < NU; ++u) {
+            if (!has[u]) continue;
+            Acc v;
+            if constexpr (SUM) v = (double)__uint_as_float(gv[u]);
+            else if constexpr (PLUS) v = (gv[u] > kInfU32 - wt[u]) ? kInfU32 : gv[u] + wt[u];
+            else v = gv[u] ^ kFlip;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+              if (q == kk[u]) {
+                if constexpr (SUM) acc[q] += v;
+                else acc[q] = v < acc[q] ? v : acc[q];
+              }
+            }
+          }
+        }
+      }
+      // totals over the tile slots, then row l of the block row to lane l
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        for (int off = LPT; off < kWave; off <<= 1) {
+          const Acc o = __shfl_xor(acc[k], off);
+          if constexpr (SUM) acc[k] += o;
+          else acc[k] = o < acc[k] ? o : acc[k];
+        }
+      }
+      const int src = (lane >> 2) & (LPT - 1);
+      Acc v0 = __shfl(acc[0], src), v1 = __shfl(acc[1], src), v2 = __shfl(acc[2], src), v3 = __shfl(acc[3], src);
+      const int kq = lane & 3;
+      const Acc m = kq == 0 ? v0 : kq == 1 ? v1 : kq == 2 ? v2 : v3;
+      const int64_t i = (int64_t)R * BS + lane;
+      if (lane < BS && i < a.nrows) {
+        if constexpr (SUM) {
+          if (full) ((float*)a.out)[i] = (float)((double)a.scale * m + (double)a.bias);
+          else a.partial[((int64_t)c * 2 + (rs < cb ? 0 : 1)) * BS + lane] = m;
+        } else if constexpr (PLUS) {
+          uint32_t* out = (uint32_t*)a.out;
+          if (full) {
+            const uint32_t dd = xv[i];
+            out[i] = m < dd ? m : dd;
+          } else if (m != kInfU32) {
+            atomicMin(&out[i], m);
+          }
+        } else {
+          int32_t* y = (int32_t*)a.out;
+          const int32_t val = (int32_t)(m ^ kFlip);
+          if (full) y[i] = val;
+          else if (val != INT32_MAX) atomicMin(&y[i], val);
+        }
+      }
+    }
+    if (__popcll(bmask) < kWave) break;
+    R0 += kWave;
+  }
+}
+
 // bs 8/16 min / min-plus, lane per tile (plan variant 3).  A wave takes its
 // chunk 64 block rows (a window) at a time and the window's tiles 64 per
 // round (one per lane: more tiles per lane made the kernel register-bound
@@ -1607,8 +1809,9 @@ __global__ __launch_bounds__(256) void k_bfs_pack(const int32_t* __restrict__ cn
 
 template <int BS, int OP>
 static int spmv_launch_t(const SpmvArgs& a, hipStream_t st) {
-  // variants 2 and 3 (streamed) read 16- or 8-byte vectors of tiles and
-  // column indices: they need 16-byte-aligned arrays, otherwise variant 0
+  // the streamed and multi-row-per-lane kernels (variants 2 and 3, the bs
+  // 32 count / min / min-plus) read 16- or 8-byte vectors of tiles and column
+  // indices: they need 16-byte-aligned arrays, otherwise the row-group kernel
   // runs.
   const bool aligned = ((((uintptr_t)a.tiles) | ((uintptr_t)a.colind)) & 15) == 0;
   // variant 3: bs 8 count is k_count8, bs 16 count k_count16q
@@ -1665,6 +1868,9 @@ static int spmv_launch_t(const SpmvArgs& a, hipStream_t st) {
   } else if constexpr (BS <= 16 && (OP == OP_MIN || OP == OP_MINPLUS || OP == OP_SUM)) {
     // variant 3 (the bs 8/16 default): the set-bit list kernel
     if (a.variant == 3 && aligned) k_value_lpt<BS, OP><<<(unsigned)blocks, 256, 0, st>>>(a);
+    else k_spmv<BS, OP><<<(unsigned)blocks, 256, 0, st>>>(a);
+  } else if constexpr (BS == 32 && (OP == OP_MIN || OP == OP_MINPLUS)) {
+    if (aligned) k_value_q<BS, OP><<<(unsigned)blocks, 256, 0, st>>>(a);
     else k_spmv<BS, OP><<<(unsigned)blocks, 256, 0, st>>>(a);
   } else if constexpr (BS == 32 && OP == OP_COUNT) {
     if (aligned) k_count32q<<<(unsigned)blocks, 256, 0, st>>>(a);

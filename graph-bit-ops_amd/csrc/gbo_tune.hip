@@ -81,17 +81,13 @@ int gbo_autotune(const int32_t* d_rowptr, const int32_t* d_colind, int32_t nrows
     const int dc = default_chunk(bs);
     const int chunks[3] = {dc, dc * 2, dc / 2};
     // kernel variants that differ for this block size and operation (bs 8
-    // BFS: 3 is the dense count-based step; 2 would run variant 0)
-    static const int kVarCount8[] = {0, 1, 2, 3}, kVarBfs8[] = {0, 1, 3}, kVarCount16[] = {0, 1, 2},
-                     kVarBfs16[] = {0, 1}, kVarOne[] = {0};
+    // and 16 BFS: 3 is the dense count-based step; 2 would run variant 0)
+    static const int kVarCount[] = {0, 1, 2, 3}, kVarBfs[] = {0, 1, 3}, kVarOne[] = {0};
     const int* vars = kVarOne;
     int nvar = 1;
-    if (bs == 8) {
-      vars = op == 0 ? kVarCount8 : kVarBfs8;
+    if (bs <= 16) {
+      vars = op == 0 ? kVarCount : kVarBfs;
       nvar = op == 0 ? 4 : 3;
-    } else if (bs == 16) {
-      vars = op == 0 ? kVarCount16 : kVarBfs16;
-      nvar = op == 0 ? 3 : 2;
     }
     for (int vi = 0; vi < nvar && rc == GBO_OK; ++vi)
       for (int ci = 0; ci < 3 && rc == GBO_OK; ++ci) {

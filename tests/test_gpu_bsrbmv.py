@@ -174,14 +174,14 @@ def test_autotune(gbo, po):
 
 
 def test_autotune_bfs_step(gbo, po):
-    """BFS-step tuning (bs 8 includes the dense variant 3): the picked plan's
-    step equals the oracle's."""
+    """BFS-step tuning (bs 8 and 16 include the dense variant 3): the picked
+    plan's step equals the oracle's."""
     g = gbo.gen_rmat(12, 16, seed=2)
     d_rp = gbo.DeviceBuffer.from_numpy(g.rowptr)
     d_ci = gbo.DeviceBuffer.from_numpy(g.colind)
     t = gbo.autotune(d_rp, d_ci, g.nrows, g.ncols, 1)
     assert t.bs in (8, 16, 32, 64) and t.chunk > 0 and t.ms > 0
-    assert t.variant in ((0, 1, 3) if t.bs == 8 else (0, 1) if t.bs == 16 else (0,))
+    assert t.variant in ((0, 1, 3) if t.bs <= 16 else (0,))
     A = gbo.BsrMatrix.from_csr(g, t.bs, plan=False)
     A.plan(t.chunk, t.variant)
     rng = np.random.default_rng(5)
@@ -191,3 +191,51 @@ def test_autotune_bfs_step(gbo, po):
     nx, c = gbo.bsrbmv_bfs_step(A, f, v)
     onx, oc = po.bfs_step(t.bs, g.nrows, rp, ci, tw, f, v)
     assert np.array_equal(nx, onx) and c == oc
+
+
+class _TileView:
+    """A gbo_bsr that shares a matrix's arrays and plan but reads its tiles
+    from another address; never freed (it owns nothing)."""
+
+    def __init__(self, gbo, A, tiles_ptr):
+        import ctypes
+        self.s = gbo.BsrStruct()
+        ctypes.memmove(ctypes.addressof(self.s), ctypes.addressof(A.s), ctypes.sizeof(self.s))
+        self.s.tiles = tiles_ptr
+        self.ref = ctypes.byref(self.s)
+        self.bs, self.nrows, self.nblockrows, self.nblocks = A.bs, A.nrows, A.nblockrows, A.nblocks
+
+
+@pytest.mark.parametrize("bs", BS)
+def test_unaligned_tiles_give_the_same_results(gbo, bs):
+    """The kernels that read tiles as 8- or 16-byte vectors (bs 8/16 variant 3,
+    the bs 32 count / min / min-plus, the dense BFS step) need 16-byte-aligned
+    tiles; tiles 8 bytes into a buffer take the row-group kernels and give the
+    same results (the FP32 sum within rounding: its order of additions
+    differs)."""
+    g = gbo.gen_rmat(11, 16, seed=4)
+    A = gbo.BsrMatrix.from_csr(g, bs)
+    nb = A.tile_bytes()
+    buf = gbo.DeviceBuffer(nb + 16)
+    gbo._check(gbo.lib().gbo_memcpy_d2d(buf.ptr + 8, A.s.tiles, nb))
+    V = _TileView(gbo, A, buf.ptr + 8)
+    rng = np.random.default_rng(11)
+    xb = gbo.pack_vec(bs, (rng.random(g.ncols) < 0.3).astype(np.float32))
+    assert np.array_equal(gbo.bsrbmv_count(V, xb), gbo.bsrbmv_count(A, xb))
+    f = gbo.pack_vec(bs, (rng.random(g.ncols) < 0.5).astype(np.float32))
+    v = gbo.pack_vec(bs, (rng.random(g.nrows) < 0.1).astype(np.float32))
+    nx, c = gbo.bsrbmv_bfs_step(V, f, v)
+    nx0, c0 = gbo.bsrbmv_bfs_step(A, f, v)
+    assert np.array_equal(nx, nx0) and c == c0
+    dist = rng.integers(0, 1000, g.nrows).astype(np.uint32)
+    dist[rng.random(g.nrows) < 0.3] = gbo.INF_U32
+    assert np.array_equal(gbo.bsrbmv_minplus(V, dist), gbo.bsrbmv_minplus(A, dist))
+    tv = gbo.TileValues(A, g, gbo.edge_weights(g, 1, 64, seed=9))
+    assert np.array_equal(gbo.bsrbmv_minplus(V, dist, tv), gbo.bsrbmv_minplus(A, dist, tv))
+    xi = rng.integers(-1000, 1000, g.ncols).astype(np.int32)
+    assert np.array_equal(gbo.bsrbmv_min_i32(V, xi), gbo.bsrbmv_min_i32(A, xi))
+    xf = rng.random(g.ncols).astype(np.float32)
+    np.testing.assert_allclose(gbo.bsrbmv_sum_f32(V, xf), gbo.bsrbmv_sum_f32(A, xf), rtol=1e-6)
+    tv.free()
+    buf.free()
+    A.free()
