@@ -1312,19 +1312,25 @@ __global__ __launch_bounds__(256) void k_count32q(const SpmvArgs a) {
 // down from the lane's last bit); the four row accumulators are summed over
 // the tile slots at the end of a block row (FP64 butterfly for the sum: a
 // fixed order) and moved to one lane per row for the write.
-// Used at bs 32 for min and min-plus (R-MAT s22: weighted min-plus 4.51 ->
-// 3.85 ms, unit 3.88 -> 3.00 ms, min 2.36 -> 2.30 ms against the row-group
-// kernel).  The FP32 sum stays on the row-group kernel (2.77 against 2.70
-// ms), and at bs 16 the set-bit list kernel is faster for every op (min 1.05
-// against 1.49 ms).
+// Used at bs 32 (R-MAT s22 against the row-group kernel: weighted min-plus
+// 4.51 -> 2.83 ms, unit 3.88 -> 2.89 ms, sum 2.70 -> 2.57 ms, min 2.36 ->
+// 2.29 ms).  Tiles per lane in flight: four for min; two for min-plus and
+// the sum, whose extra registers (weights, FP64 accumulators) cost waves at
+// four (weighted min-plus 3.85, sum 2.77 ms) and whose single tile lost
+// overlap (3.37, 3.08 ms).  At bs 16 the set-bit list kernel is faster for
+// every op (min 1.05 against 1.49 ms).  At bs 64 (sixteen lanes per tile)
+// only the weighted min-plus gains (6.77 -> 5.60 ms); min, the sum and the
+// unit-weight min-plus are faster on the row-group kernel, whose per-tile
+// reads are scalar there.
 template <int BS, int OP>
 __global__ __launch_bounds__(256) void k_value_q(const SpmvArgs a) {
-  static_assert(BS == 16 || BS == 32, "four rows per lane at bs 16 or 32");
+  static_assert(BS == 16 || BS == 32 || BS == 64, "four rows per lane");
+  using RW = typename std::conditional<BS == 64, uint64_t, uint32_t>::type;  // row word
   static_assert(OP == OP_MIN || OP == OP_MINPLUS || OP == OP_SUM, "min / min-plus / sum only");
   constexpr bool PLUS = OP == OP_MINPLUS;
   constexpr bool SUM = OP == OP_SUM;
   using Acc = typename std::conditional<SUM, double, uint32_t>::type;
-  constexpr int LPT = BS / 4, SL = kWave / LPT, NU = 4;
+  constexpr int LPT = BS / 4, SL = kWave / LPT, NU = OP == OP_MIN ? 4 : 2;
   constexpr uint32_t kFlip = 0x80000000u;
   const Acc ident = SUM ? (Acc)0 : (Acc)0xFFFFFFFFu;
   const int lane = lane_id();
@@ -1360,7 +1366,7 @@ __global__ __launch_bounds__(256) void k_value_q(const SpmvArgs a) {
       for (int64_t t0 = s; t0 < te; t0 += (int64_t)SL * NU) {
         int32_t cc[NU];
         bool act[NU];
-        uint32_t rw[NU][4];
+        RW rw[NU][4];
         int rank[NU];
         int64_t wo[NU];
         // three stages over the NU tiles so each stage's loads are in flight
@@ -1385,10 +1391,13 @@ __global__ __launch_bounds__(256) void k_value_q(const SpmvArgs a) {
         for (int u = 0; u < NU; ++u) {
           const int64_t t = t0 + (int64_t)u * SL + slot;
           const int64_t tc = t < te ? t : s;
-          uint32_t r4[4] = {0u, 0u, 0u, 0u};
+          RW r4[4] = {0, 0, 0, 0};
           wo[u] = 0;
           if (!PLUS || act[u]) {
-            if constexpr (BS == 32) {
+            if constexpr (BS == 64) {
+              const uint64_t* tp = (const uint64_t*)a.tiles + tc * 64 + 4 * p;
+              r4[0] = tp[0]; r4[1] = tp[1]; r4[2] = tp[2]; r4[3] = tp[3];
+            } else if constexpr (BS == 32) {
               typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
               const u32x4 v = ((const u32x4*)a.tiles)[tc * 8 + p];
               r4[0] = v.x; r4[1] = v.y; r4[2] = v.z; r4[3] = v.w;
@@ -1402,14 +1411,14 @@ __global__ __launch_bounds__(256) void k_value_q(const SpmvArgs a) {
             if (weighted) wo[u] = a.woff[tc];
           }
 #pragma unroll
-          for (int k = 0; k < 4; ++k) rw[u][k] = act[u] ? r4[k] : 0u;
+          for (int k = 0; k < 4; ++k) rw[u][k] = act[u] ? r4[k] : (RW)0;
         }
         if (weighted) {
           // weight rank of the lane's last set bit: set bits of the tile's
           // earlier rows (lanes p' < p of the slot) plus the lane's own
 #pragma unroll
           for (int u = 0; u < NU; ++u) {
-            const int pc = __popc(rw[u][0]) + __popc(rw[u][1]) + __popc(rw[u][2]) + __popc(rw[u][3]);
+            const int pc = popcw(rw[u][0]) + popcw(rw[u][1]) + popcw(rw[u][2]) + popcw(rw[u][3]);
             int incl = pc;
 #pragma unroll
             for (int off = 1; off < LPT; off <<= 1) {
@@ -1422,7 +1431,7 @@ __global__ __launch_bounds__(256) void k_value_q(const SpmvArgs a) {
         while (true) {
           bool any = false;
 #pragma unroll
-          for (int u = 0; u < NU; ++u) any |= (rw[u][0] | rw[u][1] | rw[u][2] | rw[u][3]) != 0u;
+          for (int u = 0; u < NU; ++u) any |= (rw[u][0] | rw[u][1] | rw[u][2] | rw[u][3]) != (RW)0;
           if (!__ballot(any)) break;  // wave-uniform
           // one set bit of each tile; all gathers of the pass are issued
           // before any is used
@@ -1431,18 +1440,18 @@ __global__ __launch_bounds__(256) void k_value_q(const SpmvArgs a) {
           bool has[NU];
 #pragma unroll
           for (int u = 0; u < NU; ++u) {
-            const uint32_t w0 = rw[u][0], w1 = rw[u][1], w2 = rw[u][2], w3 = rw[u][3];
+            const RW w0 = rw[u][0], w1 = rw[u][1], w2 = rw[u][2], w3 = rw[u][3];
             const int k = w3 ? 3 : w2 ? 2 : w1 ? 1 : 0;
-            const uint32_t wk = k == 3 ? w3 : k == 2 ? w2 : k == 1 ? w1 : w0;
-            has[u] = wk != 0u;
+            const RW wk = k == 3 ? w3 : k == 2 ? w2 : k == 1 ? w1 : w0;
+            has[u] = wk != (RW)0;
             kk[u] = k;
             gv[u] = 0u;
             wt[u] = 1u;
             if (has[u]) {
-              const int b = __builtin_ctz(wk);
+              const int b = BS == 64 ? __builtin_ctzll((uint64_t)wk) : __builtin_ctz((uint32_t)wk);
               gv[u] = xv[(int64_t)cc[u] * BS + (BS - 1 - b)];
               if (weighted) wt[u] = a.w[wo[u] + rank[u]];
-              const uint32_t nw = wk & (wk - 1u);
+              const RW nw = wk & (wk - (RW)1);
 #pragma unroll
               for (int q = 0; q < 4; ++q) rw[u][q] = q == k ? nw : rw[u][q];
               --rank[u];
@@ -1869,8 +1878,11 @@ static int spmv_launch_t(const SpmvArgs& a, hipStream_t st) {
     // variant 3 (the bs 8/16 default): the set-bit list kernel
     if (a.variant == 3 && aligned) k_value_lpt<BS, OP><<<(unsigned)blocks, 256, 0, st>>>(a);
     else k_spmv<BS, OP><<<(unsigned)blocks, 256, 0, st>>>(a);
-  } else if constexpr (BS == 32 && (OP == OP_MIN || OP == OP_MINPLUS)) {
+  } else if constexpr (BS == 32 && (OP == OP_MIN || OP == OP_MINPLUS || OP == OP_SUM)) {
     if (aligned) k_value_q<BS, OP><<<(unsigned)blocks, 256, 0, st>>>(a);
+    else k_spmv<BS, OP><<<(unsigned)blocks, 256, 0, st>>>(a);
+  } else if constexpr (BS == 64 && OP == OP_MINPLUS) {
+    if (aligned && a.w) k_value_q<BS, OP><<<(unsigned)blocks, 256, 0, st>>>(a);
     else k_spmv<BS, OP><<<(unsigned)blocks, 256, 0, st>>>(a);
   } else if constexpr (BS == 32 && OP == OP_COUNT) {
     if (aligned) k_count32q<<<(unsigned)blocks, 256, 0, st>>>(a);
