@@ -116,6 +116,33 @@ def time_bfs_step(gbo, A, dx, steps, warmup, timer):
     return ms
 
 
+def pick_value_plan(gbo, A, count_plan, steps, warmup, timer):
+    """(chunk, variant) for the set-bit drivers (PageRank, SSSP, CC): the
+    count-tuned plan or a default chunk (and at bs 8/16 variant 3, the set-bit
+    list kernel), whichever gives the fastest FP32 sum sweep.  Plans never
+    change integer results; the sum's order of additions follows the chunk."""
+    L = gbo.lib()
+    n = A.nrows
+    bc, bv = count_plan
+    dc = gbo_default_chunk(A.bs)
+    cands = {(c, v) for c in (bc, dc, 2 * dc) for v in ({bv, 3} if A.bs <= 16 else {bv})}
+    xs = gbo.DeviceBuffer.from_numpy(np.full(n, 1.0 / n, np.float32))
+    dy = gbo.DeviceBuffer(4 * max(n, 1))
+    times = {}
+    for c, v in sorted(cands):
+        A.plan(c, v)
+        for _ in range(warmup):
+            gbo._check(L.gbo_bsrbmv_sum_f32(A.ref, xs.ptr, dy.ptr, None))
+        gbo.synchronize()
+        timer.start()
+        for _ in range(steps):
+            gbo._check(L.gbo_bsrbmv_sum_f32(A.ref, xs.ptr, dy.ptr, None))
+        times[(A.s.plan_chunk, v)] = timer.stop_ms() / steps
+    xs.free()
+    dy.free()
+    return min(times, key=times.get), times
+
+
 def time_value_sweeps(gbo, A, g, w_sssp, steps, warmup, timer):
     """One full sweep of each value op over every tile: sum (PageRank's
     product, x = 1/n), min (CC's product, x = vertex ids) and min-plus
@@ -651,6 +678,12 @@ def main():
                           "direction_optimizing_ms": round(do_s * 1e3, 2),
                           "direction_optimizing_graph500_GTEPS": round(edges / do_s / 1e9, 3),
                           "direction_optimizing_equal": bool(np.array_equal(lv_do, lv))}
+            # PageRank, SSSP and CC run on the plan fastest for the set-bit
+            # kernels (timed with the sum sweep); BFS above used the count's
+            (vc, vv), vtimes = pick_value_plan(gbo, A, (bc, bv), args.steps, args.warmup, timer)
+            A.plan(vc, vv)
+            rec["value_plan"] = {"chunk": vc, "variant": vv,
+                                 "sum_ms_by_plan": {f"{c},{v}": round(t, 4) for (c, v), t in vtimes.items()}}
             pr_s, (_, it, delta) = timed(lambda: gbo.pagerank(A, d_od, 0.85, 1e-7, 1000, out=d_res))
             # per iteration: one sum sweep (tiles, colind, rowptr, read p, write p)
             pr_bytes = sweep_bytes(A, n) - 4 * n + 8 * n
@@ -671,7 +704,9 @@ def main():
             d_od.free()
             rec["cc"] = {"iters": cit, "components": int(len(np.unique(lab))), "ms": round(cc_s * 1e3, 1)}
             rec["value_sweeps"] = time_value_sweeps(gbo, A, g, w_sssp, max(3, args.steps // 4), 1, timer)
-            log(f"[bench] bs={bs} bfs {rec['bfs']} pr {rec['pagerank']} sssp {rec['sssp']} cc {rec['cc']}")
+            A.plan(bc, bv)  # back to the count-tuned plan
+            log(f"[bench] bs={bs} bfs {rec['bfs']} pr {rec['pagerank']} sssp {rec['sssp']} cc {rec['cc']} "
+                f"value plan {rec['value_plan']}")
         per_bs[bs] = rec
         keep[bs] = (A, dx)
         if len(keep) > 1:  # keep only the best so far resident
