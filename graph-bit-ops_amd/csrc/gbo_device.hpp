@@ -86,7 +86,8 @@ __device__ __forceinline__ uint64_t fold_groups(uint64_t b) {
   if constexpr (BS <= 32) b |= b >> 32;
   if constexpr (BS <= 16) b |= b >> 16;
   if constexpr (BS <= 8) b |= b >> 8;
-  return BS == 64 ? b : (b & ((1ull << BS) - 1ull));
+  if constexpr (BS == 64) return b;
+  else return b & ((1ull << BS) - 1ull);
 }
 
 // Mask with the low BS bits set.
