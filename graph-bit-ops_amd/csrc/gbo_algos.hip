@@ -181,9 +181,9 @@ static int bfs_t(const gbo_bsr* A, const gbo_bsr* Aout, int32_t source, int32_t*
   a.nchunks = pv.nchunks;
   a.variant = pv.variant;
   a.count_slots = pv.slots;
-  // bs 8/16 variant 3 and bs 32: levels while fewer than half the vertices
-  // are visited take the dense step (count kernel + pack); later ones skip
-  // visited rows
+  // bs 8/16 variant 3 and bs 32/64: levels while fewer than half the
+  // vertices are visited take the dense step (count kernel + pack); later
+  // ones skip visited rows
   int32_t* dense_counts = dense_bfs_step(BS, pv.variant) ? pv.bfs_counts : nullptr;
   int64_t visited = 1;
   int32_t lvl = 0;

@@ -1787,7 +1787,7 @@ __global__ __launch_bounds__(256) void k_sum_fixup(const SpmvArgs a) {
   if (i < a.nrows) ((float*)a.out)[i] = (float)((double)a.scale * s + (double)a.bias);
 }
 
-// Dense bs 8/16/32 BFS step, second half: next[R] = rows of block row R whose
+// Dense BFS step, second half: next[R] = rows of block row R whose
 // frontier-neighbour count is nonzero, minus the visited ones; new bits are
 // counted into the slots.
 template <int BS>
@@ -1795,7 +1795,7 @@ __global__ __launch_bounds__(256) void k_bfs_pack(const int32_t* __restrict__ cn
                                                   word_t<BS>* __restrict__ next, int32_t nbr, int32_t nrows,
                                                   unsigned long long* __restrict__ slots) {
   const int64_t R = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  uint32_t word = 0;
+  uint64_t word = 0;
   if (R < nbr) {
     const int4* p = (const int4*)(cnt + R * BS);
 #pragma unroll
@@ -1805,13 +1805,13 @@ __global__ __launch_bounds__(256) void k_bfs_pack(const int32_t* __restrict__ cn
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
         const int r = 4 * h + k;
-        if (R * BS + r < nrows && c4[k] > 0) word |= 1u << (BS - 1 - r);
+        if (R * BS + r < nrows && c4[k] > 0) word |= 1ull << (BS - 1 - r);
       }
     }
-    word &= ~(uint32_t)vis[R];
+    word &= ~(uint64_t)vis[R];
     next[R] = (word_t<BS>)word;
   }
-  uint64_t added = (uint64_t)__popc(word);
+  uint64_t added = (uint64_t)__popcll(word);
   for (int off = 1; off < kWave; off <<= 1) added += __shfl_xor(added, off);
   if (lane_id() == 0 && added) atomicAdd(&slots[(R / kWave) & (kCountSlots - 1)], (unsigned long long)added);
 }
@@ -1832,11 +1832,11 @@ static int spmv_launch_t(const SpmvArgs& a, hipStream_t st) {
   const bool lpt = ((BS <= 16 && (OP == OP_COUNT || OP == OP_BFS) && a.variant == 1) ||
                     (BS == 8 && OP == OP_BFS && a.variant == 3)) &&
                    !(OP == OP_BFS && a.sparse_frontier);
-  if constexpr (BS <= 32 && OP == OP_BFS) {
+  if constexpr (OP == OP_BFS) {
     if (dense_bfs_step(BS, a.variant) && a.bfs_counts && aligned && !a.sparse_frontier) {
-      // dense step: the count kernel (k_count8 / k_count16q / k_count32q)
-      // gives each row's number of frontier neighbours (every tile is read;
-      // no visited-row skip)
+      // dense step: the count kernel (k_count8 / k_count16q / k_count32q /
+      // k_spmv<64>) gives each row's number of frontier neighbours (every
+      // tile is read; no visited-row skip)
       SpmvArgs c = a;
       c.out = a.bfs_counts;
       c.accumulate = 0;

@@ -20,7 +20,6 @@ namespace gbo {
 //   count       1 x int64           the folded frontier size
 //   partial     nchunks*2*bs x f64  split-row partials of the FP32 sum
 //   bfs_counts  bs*(nbr+1) x int32  per-row counts of the dense BFS step
-//                                   (bs 8, 16 and 32 only)
 // Calls on one matrix therefore share that scratch: they must be ordered
 // (one stream, or synchronised); separately created matrices do not share.
 struct PlanLayout {
@@ -37,7 +36,7 @@ struct PlanView {
   unsigned long long* slots = nullptr;
   int64_t* count = nullptr;
   double* partial = nullptr;
-  int32_t* bfs_counts = nullptr;  // null unless bs 8, 16 or 32
+  int32_t* bfs_counts = nullptr;  // the dense BFS step's row counts
 };
 
 int default_chunk(int bs);
@@ -99,8 +98,8 @@ enum SpmvOp { OP_COUNT = 0, OP_BFS = 1, OP_MINPLUS = 2, OP_SUM = 3, OP_MIN = 4 }
 
 // Whether a BFS step at this block size and plan variant may take the dense
 // form (count kernel into per-row counts, then a pack of the nonzero rows):
-// bs 8/16 under variant 3, bs 32 under every variant.
-inline bool dense_bfs_step(int bs, int variant) { return (bs <= 16 && variant == 3) || bs == 32; }
+// bs 8/16 under variant 3, bs 32/64 under every variant.
+inline bool dense_bfs_step(int bs, int variant) { return (bs <= 16 && variant == 3) || bs >= 32; }
 
 struct SpmvArgs {
   const int32_t* rowptr;

@@ -473,10 +473,8 @@ PlanLayout plan_layout(int bs, int64_t nchunks, int32_t nbr, bool scratch) {
     off += 256;
     L.partial = off;
     off += align256(sizeof(double) * 2 * (size_t)nchunks * bs);
-    if (bs <= 32) {
-      L.bfs_counts = off;
-      off += align256(sizeof(int32_t) * (size_t)bs * ((size_t)nbr + 1));
-    }
+    L.bfs_counts = off;
+    off += align256(sizeof(int32_t) * (size_t)bs * ((size_t)nbr + 1));
   }
   L.bytes = off;
   return L;
@@ -489,7 +487,7 @@ static void view_scratch(PlanView* pv, int bs, int32_t nbr) {
   pv->slots = (unsigned long long*)(base + L.slots);
   pv->count = (int64_t*)(base + L.count);
   pv->partial = (double*)(base + L.partial);
-  pv->bfs_counts = bs <= 32 ? (int32_t*)(base + L.bfs_counts) : nullptr;
+  pv->bfs_counts = (int32_t*)(base + L.bfs_counts);
 }
 
 // Plan entries plus (optionally) zeroed count slots in one allocation.

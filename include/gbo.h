@@ -228,8 +228,9 @@ int gbo_bsr_plan(gbo_bsr* A, int32_t chunk_tiles, gbo_stream_t stream);
  * visited-row-skipping step after); at bs 8 and 16, min / min-plus / FP32 sum
  * list the tiles' set bits in LDS so the value gathers spread over all lanes
  * (the sum folds them into FP64 accumulators private to one wavefront).
- * bs 32 has one kernel set whatever the variant: COUNT with eight lanes per
- * tile (16 bytes each) and the dense BFS step on it.
+ * bs 32 and 64 have one kernel set whatever the variant; their BFS step is
+ * dense too (bs 32 COUNT with eight lanes per tile, bs 64 COUNT with scalar
+ * per-tile reads).
  * Integer results are identical; only speed differs (the autotuner picks).
  * The FP32 sum's order of additions depends on the chunk size and variant,
  * each fixed by the plan, so its last bits may differ between plans.
@@ -353,7 +354,7 @@ int gbo_bfs(const gbo_bsr* AT, int32_t source, int32_t* d_level,
  * in one static device array of 32 MiB held by each GPU the library runs on
  * until the process ends; a graph whose scratch exceeds it (about 89M
  * vertices at any block size) is refused with GBO_ERANGE before any launch.
- * At bs 8/16 (variant 3) and 32 the dense step's row counts (4 bytes a vertex) go
+ * At bs 8/16 (variant 3), 32 and 64 the dense step's row counts (4 bytes a vertex) go
  * there too for graphs up to about 7.6M vertices; larger ones take the
  * visited-row-skipping step at every level (same levels).
  * Calls that overlap on one GPU share that array and race: order them (one
