@@ -1135,12 +1135,12 @@ __device__ __forceinline__ uint32_t row_of_quad(const uint32_t* c, int lane) {
 // instruction) and rows 4q..4q+3 (q = l % 4) of it as one 8-byte word; x is
 // replicated into the four 16-bit fields and each field's popcount added to
 // the lane's row counter, which are summed over the 16 slots at the end of a
-// block row.  Against the row-group kernel (16 lanes of 2 bytes per tile):
-// R-MAT s22 0.884 ms against 0.954, same counts; two lanes per tile (8
-// counters per lane) took 0.996.  The row totals are moved to lanes 0..15
-// before they are written: four strided store or atomic instructions per
-// block row had tripled the written bytes (split rows' atomics are not
-// coalesced).
+// block row.  The row totals are moved to lanes 0..15 before they are
+// written: four strided store or atomic instructions per block row had
+// tripled the written bytes (split rows' atomics are not coalesced).
+// Against the row-group kernel (16 lanes of 2 bytes per tile): R-MAT s22
+// 0.79-0.80 ms against 0.94-0.96, same counts (0.88 before the transposed
+// write); two lanes per tile (8 counters per lane) took 0.996.
 constexpr int kCount16Step = 4;  // tiles per lane in flight
 __global__ __launch_bounds__(256) void k_count16q(const SpmvArgs a) {
   constexpr int BS = 16, NU = kCount16Step, SL = kWave / 4;  // tile slots per step
@@ -1223,8 +1223,9 @@ __global__ __launch_bounds__(256) void k_count16q(const SpmvArgs a) {
 // 4p..4p+3, p = l % 8) of tile slot l / 8, so one wave instruction reads
 // eight whole tiles (1 KB contiguous) and one column-index load and one x
 // gather instruction serve them (the row-group kernel needs one of each per
-// two tiles).  R-MAT s22: 1.58 ms against 1.64; eight tiles per lane in
-// flight instead of four: 1.59 ms (74 VGPRs).
+// two tiles); row totals are written as in k_count16q.  R-MAT s22: 1.53-1.56
+// ms against 1.64; eight tiles per lane in flight instead of four was no
+// faster (74 VGPRs).
 constexpr int kCount32Step = 4;  // tiles per lane in flight
 __global__ __launch_bounds__(256) void k_count32q(const SpmvArgs a) {
   constexpr int BS = 32, NU = kCount32Step, SL = kWave / 8;  // tile slots per step
