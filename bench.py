@@ -32,7 +32,7 @@ METRIC = "bsrbmv GTEPS + HBM roofline frac, R-MAT s22; batched/adjoint vs plain 
 
 
 def gbo_default_chunk(bs):
-    return {8: 256, 16: 256, 32: 256, 64: 128}[bs]
+    return {8: 256, 16: 256, 32: 256, 64: 256}[bs]
 
 
 PROFILE_MD = os.path.join(ROOT, "profiles", "bsrbmv_count_rmat_s22.md")

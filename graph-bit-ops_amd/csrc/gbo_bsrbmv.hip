@@ -1223,10 +1223,10 @@ __global__ __launch_bounds__(256) void k_count16q(const SpmvArgs a) {
 // 4p..4p+3, p = l % 8) of tile slot l / 8, so one wave instruction reads
 // eight whole tiles (1 KB contiguous) and one column-index load and one x
 // gather instruction serve them (the row-group kernel needs one of each per
-// two tiles); row totals are written as in k_count16q.  R-MAT s22: 1.53-1.56
-// ms against 1.64; eight tiles per lane in flight instead of four was no
-// faster (74 VGPRs).
-constexpr int kCount32Step = 4;  // tiles per lane in flight
+// two tiles); row totals are written as in k_count16q.  R-MAT s22: 1.50 ms
+// against 1.64 with two tiles per lane in flight (1.52-1.54 with four, 1.53-
+// 1.55 with eight).
+constexpr int kCount32Step = 2;  // tiles per lane in flight
 __global__ __launch_bounds__(256) void k_count32q(const SpmvArgs a) {
   constexpr int BS = 32, NU = kCount32Step, SL = kWave / 8;  // tile slots per step
   const int lane = lane_id();
