@@ -116,34 +116,29 @@ def time_bfs_step(gbo, A, dx, steps, warmup, timer):
     return ms
 
 
-def pick_value_plan(gbo, A, count_plan, steps, warmup, timer):
-    """(chunk, variant) for the set-bit drivers (PageRank, SSSP, CC): the
-    count-tuned plan or a default chunk (and at bs 8/16 variant 3, the set-bit
-    list kernel), whichever gives the fastest FP32 sum sweep.  Plans never
-    change integer results; the sum's order of additions follows the chunk."""
-    L = gbo.lib()
-    n = A.nrows
+def pick_plan(gbo, A, count_plan, run, steps, warmup, timer):
+    """(chunk, variant) for one set-bit driver: the count-tuned plan or a
+    default chunk (and at bs 8/16 variant 3, the set-bit list kernel),
+    whichever gives the fastest sweep of that driver's product `run`.  Plans
+    never change integer results; the sum's order of additions follows the
+    chunk."""
     bc, bv = count_plan
     dc = gbo_default_chunk(A.bs)
     cands = {(c, v) for c in (bc, dc, 2 * dc) for v in ({bv, 3} if A.bs <= 16 else {bv})}
-    xs = gbo.DeviceBuffer.from_numpy(np.full(n, 1.0 / n, np.float32))
-    dy = gbo.DeviceBuffer(4 * max(n, 1))
     times = {}
     for c, v in sorted(cands):
         A.plan(c, v)
         for _ in range(warmup):
-            gbo._check(L.gbo_bsrbmv_sum_f32(A.ref, xs.ptr, dy.ptr, None))
+            gbo._check(run())
         gbo.synchronize()
         timer.start()
         for _ in range(steps):
-            gbo._check(L.gbo_bsrbmv_sum_f32(A.ref, xs.ptr, dy.ptr, None))
+            gbo._check(run())
         times[(A.s.plan_chunk, v)] = timer.stop_ms() / steps
-    xs.free()
-    dy.free()
     return min(times, key=times.get), times
 
 
-def time_value_sweeps(gbo, A, g, w_sssp, steps, warmup, timer):
+def time_value_sweeps(gbo, A, g, w_sssp, steps, warmup, timer, plans=None):
     """One full sweep of each value op over every tile: sum (PageRank's
     product, x = 1/n), min (CC's product, x = vertex ids) and min-plus
     (SSSP's product with the uint32 [1,64] weights, half the distances
@@ -169,6 +164,8 @@ def time_value_sweeps(gbo, A, g, w_sssp, steps, warmup, timer):
                              base + 8 * n + 4 * g.nnz + 8 * A.nblocks),
     }
     for name, (fn, byt) in calls.items():
+        if plans:  # each op on its own plan
+            A.plan(*plans[name])
         for _ in range(warmup):
             gbo._check(fn())
         gbo.synchronize()
@@ -678,32 +675,49 @@ def main():
                           "direction_optimizing_ms": round(do_s * 1e3, 2),
                           "direction_optimizing_graph500_GTEPS": round(edges / do_s / 1e9, 3),
                           "direction_optimizing_equal": bool(np.array_equal(lv_do, lv))}
-            # PageRank, SSSP and CC run on the plan fastest for the set-bit
-            # kernels (timed with the sum sweep); BFS above used the count's
-            (vc, vv), vtimes = pick_value_plan(gbo, A, (bc, bv), args.steps, args.warmup, timer)
-            A.plan(vc, vv)
-            rec["value_plan"] = {"chunk": vc, "variant": vv,
-                                 "sum_ms_by_plan": {f"{c},{v}": round(t, 4) for (c, v), t in vtimes.items()}}
+            # PageRank, SSSP and CC each run on the plan fastest for their own
+            # product (sum, weighted min-plus, min); BFS above used the count's
+            if w_sssp is None:
+                w_sssp = gbo.edge_weights(g, 1, 64, 1)
+            tv = gbo.TileValues(A, g, w_sssp)
+            L = gbo.lib()
+            xs = gbo.DeviceBuffer.from_numpy(np.full(n, 1.0 / n, np.float32))
+            xi = gbo.DeviceBuffer.from_numpy(np.arange(n, dtype=np.int32))
+            xd = gbo.DeviceBuffer.from_numpy(np.where(np.arange(n) % 2 == 0, np.arange(n) % 1000,
+                                                      0xFFFFFFFF).astype(np.uint32))
+            dyv = gbo.DeviceBuffer(4 * max(n, 1))
+            products = {
+                "sum_f32": lambda: L.gbo_bsrbmv_sum_f32(A.ref, xs.ptr, dyv.ptr, None),
+                "minplus_weighted": lambda: L.gbo_bsrbmv_minplus(A.ref, tv.tval.ptr, tv.woff.ptr, xd.ptr, dyv.ptr, None),
+                "min_i32": lambda: L.gbo_bsrbmv_min_i32(A.ref, xi.ptr, dyv.ptr, None),
+            }
+            vplans, rec["value_plan"] = {}, {}
+            for op, run in products.items():
+                vplans[op], times = pick_plan(gbo, A, (bc, bv), run, args.steps, args.warmup, timer)
+                rec["value_plan"][op] = {"chunk": vplans[op][0], "variant": vplans[op][1],
+                                         "ms_by_plan": {f"{c},{v}": round(t, 4) for (c, v), t in times.items()}}
+            for b in (xs, xi, xd, dyv):
+                b.free()
+            A.plan(*vplans["sum_f32"])
             pr_s, (_, it, delta) = timed(lambda: gbo.pagerank(A, d_od, 0.85, 1e-7, 1000, out=d_res))
             # per iteration: one sum sweep (tiles, colind, rowptr, read p, write p)
             pr_bytes = sweep_bytes(A, n) - 4 * n + 8 * n
             rec["pagerank"] = {"iters": it, "ms": round(pr_s * 1e3, 1), "ms_per_iter": round(pr_s * 1e3 / it, 3),
                                "last_l1_delta": delta, "bytes_per_iter": pr_bytes,
                                "frac_hbm_per_iter": round(pr_bytes / (pr_s / it) / 1e9 / HBM_PEAK_GBS, 4)}
-            if w_sssp is None:
-                w_sssp = gbo.edge_weights(g, 1, 64, 1)
-            tv = gbo.TileValues(A, g, w_sssp)
+            A.plan(*vplans["minplus_weighted"])
             sssp_s, (_, sit) = timed(lambda: gbo.sssp(A, src, tv, out=d_res))
             dist = d_res.to_numpy(np.uint32, n)
             tv.free()
             rec["sssp"] = {"source": src, "weights": "uint32 uniform [1,64], seed 1", "iters": sit,
                            "reached": int((dist != 0xFFFFFFFF).sum()), "ms": round(sssp_s * 1e3, 1)}
+            A.plan(*vplans["min_i32"])
             cc_s, (_, cit) = timed(lambda: gbo.cc(A, out=d_res))
             lab = d_res.to_numpy(np.int32, n)
             d_res.free()
             d_od.free()
             rec["cc"] = {"iters": cit, "components": int(len(np.unique(lab))), "ms": round(cc_s * 1e3, 1)}
-            rec["value_sweeps"] = time_value_sweeps(gbo, A, g, w_sssp, max(3, args.steps // 4), 1, timer)
+            rec["value_sweeps"] = time_value_sweeps(gbo, A, g, w_sssp, max(3, args.steps // 4), 1, timer, vplans)
             A.plan(bc, bv)  # back to the count-tuned plan
             log(f"[bench] bs={bs} bfs {rec['bfs']} pr {rec['pagerank']} sssp {rec['sssp']} cc {rec['cc']} "
                 f"value plan {rec['value_plan']}")
