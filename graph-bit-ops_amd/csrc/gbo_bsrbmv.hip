@@ -69,14 +69,6 @@ __device__ __forceinline__ word_t<BS> ld_value_tile(const word_t<BS>* p) {
   else return *p;
 }
 
-// Read of p[i] for a wave-uniform i from memory no thread writes during the
-// kernel: through a constant-address-space pointer the load is a scalar
-// (s_load) one, off the vector memory path.
-template <typename T>
-__device__ __forceinline__ T ld_uniform(const T* p, int64_t i) {
-  return ((const __attribute__((address_space(4))) T*)p)[i];
-}
-
 // Per-tile (or per-block-column) read at a tile index that is wave-uniform
 // at bs 64 (one tile per wave instruction): a scalar load there, a vector
 // load at the smaller block sizes, where a wave spans several tiles.

@@ -78,6 +78,14 @@ __device__ __forceinline__ void wave_flag(bool any, unsigned long long* flag) {
   if (__ballot(any) && (threadIdx.x & (kWave - 1)) == 0 && *flag == 0ull) *flag = 1ull;
 }
 
+// Read of p[i] for a wave-uniform i from memory no thread writes during the
+// kernel: through a constant-address-space pointer the load is a scalar
+// (s_load) one, off the vector memory path.
+template <typename T>
+__device__ __forceinline__ T ld_uniform(const T* p, int64_t i) {
+  return ((const __attribute__((address_space(4))) T*)p)[i];
+}
+
 // OR of the 64/BS groups of BS bits of a ballot: bit r is set when any lane
 // with lane % BS == r had its bit set (a row's hit over the lane groups that
 // hold it), by scalar shifts instead of a cross-lane reduction.
