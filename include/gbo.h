@@ -210,7 +210,7 @@ int gbo_bsr_create(int bs, int dir, const int32_t* h_rowptr, const int32_t* h_co
                    int32_t nrows, int32_t ncols, gbo_bsr* out, gbo_stream_t stream);
 void gbo_bsr_free(gbo_bsr* A);
 /* Work decomposition: split the tiles into chunks of `chunk_tiles` (0 = the
- * default for the block size) and record each chunk's first block row.
+ * default, 256 tiles) and record each chunk's first block row.
  * Allocates A->plan (released by gbo_bsr_free / gbo_bsr_plan_free). */
 int gbo_bsr_plan(gbo_bsr* A, int32_t chunk_tiles, gbo_stream_t stream);
 /* As gbo_bsr_plan, also selecting the bs 8/16 kernel variant of the COUNT
