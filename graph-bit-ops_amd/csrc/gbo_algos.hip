@@ -193,7 +193,7 @@ static int bfs_t(const gbo_bsr* A, const gbo_bsr* Aout, int32_t source, int32_t*
   int64_t fsize = 1;  // vertices in the current frontier
   if (sizes && max_levels > 0) sizes[0] = 1;
   while (true) {
-    GBO_HIP_CHECK(hipMemsetAsync(d_cnt, 0, sizeof(int64_t), st));
+    // both step forms end with count_fold, which writes d_cnt
     if (Aout && fsize <= std::max<int64_t>(n / 64, 1)) {
       // push: only the frontier rows' tiles of A are read
       GBO_HIP_CHECK(hipMemsetAsync(nx, 0, sizeof(W) * (size_t)std::max(nbr, 1), st));
