@@ -218,6 +218,10 @@ static int bfs_t(const gbo_bsr* A, const gbo_bsr* Aout, int32_t source, int32_t*
       if ((rc = spmv_launch(BS, OP_BFS, a, st))) return rc;
       if ((rc = count_fold(pv.slots, d_cnt, st))) return rc;
     }
+    // the level update needs no count (an empty next frontier changes
+    // nothing), so it is queued before the host waits for the count
+    k_bfs_update<BS><<<(unsigned)((nbr + 255) / 256), 256, 0, st>>>(nx, vis, d_level, nbr, n, lvl + 1);
+    GBO_LAUNCH_CHECK();
     int64_t cnt = 0;
     GBO_HIP_CHECK(hipMemcpyAsync(&cnt, d_cnt, sizeof(int64_t), hipMemcpyDeviceToHost, st));
     GBO_HIP_CHECK(hipStreamSynchronize(st));
@@ -226,8 +230,6 @@ static int bfs_t(const gbo_bsr* A, const gbo_bsr* Aout, int32_t source, int32_t*
     fsize = cnt;
     visited += cnt;
     if (sizes && lvl < max_levels) sizes[lvl] = cnt;
-    k_bfs_update<BS><<<(unsigned)((nbr + 255) / 256), 256, 0, st>>>(nx, vis, d_level, nbr, n, lvl);
-    GBO_LAUNCH_CHECK();
     std::swap(f, nx);
   }
   if (nlevels) *nlevels = lvl + 1;
