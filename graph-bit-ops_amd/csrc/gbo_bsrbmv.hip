@@ -1297,7 +1297,7 @@ __global__ __launch_bounds__(256) void k_count32q(const SpmvArgs a) {
 
 // Value products (min / min-plus / FP32 sum) with four rows per lane: lane l
 // holds rows 4p..4p+3 (p = l % (bs/4)) of tile slot l / (bs/4), so a wave
-// instruction loads 64/(bs/4) whole tiles (bs 32: 8, bs 16: 16) and a gather
+// instruction loads 64/(bs/4) whole tiles (bs 16: 16, 32: 8, 64: 4) and a gather
 // pass issues one gather instruction per four tiles per lane instead of one
 // per tile slot of the row-group kernel.  Per pass each lane takes one set
 // bit of each of its NU tiles (rows from the last to the first, each row's
